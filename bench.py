@@ -1,0 +1,175 @@
+#!/usr/bin/env python3
+"""Benchmark: SNP-pair F-tests/sec of the exhaustive pairwise epistasis scan on one MI355X.
+
+Workload (default c3): N=8000 individuals, M=50000 SNPs on one chromosome,
+MAF ~ U(0.05,0.5), no missing calls, phenotype (Gamma(2,1)-2)/sqrt(2) with 10
+planted interacting pairs (+0.15 SD in one 3x3 cell), 8df + 4df tests at the
+reference's default thresholds 6.5 / 10.5. One step = one full scan of all
+M(M-1)/2 = 1,249,975,000 pairs with the genotypes and phenotype resident on
+the device: tile list, pair kernels, hit compaction and hit read-back.
+
+Prints one JSON line on stdout; progress goes to stderr. Besides the headline
+(MFMA contraction form) it times one scan of the plain popcount form, checks
+a block of pairs against the FP64 oracle, and times the FP32 CPU restatement
+of the reference kernel on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "SNP-pair F-tests/sec at N=8000,M=50000; MFMA vs popcount form, F relerr≤1e-5"
+INT8_PEAK_TOPS = 5000.0  # MI355X dense INT8 MFMA, TOP/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(wl, seconds, threads):
+    """FP32 CPU restatement of the reference kernel (oracle/epi_refscan32.c) on seeded 128x128 tiles."""
+    import oracle as O
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    rng = np.random.default_rng(12345)
+    T = 128
+    nt = wl.nsnp // T
+
+    def tiles(k):
+        out = []
+        while len(out) < k:
+            a, b = rng.integers(0, nt, 2)
+            if a > b:
+                out.append((a * T, b * T))
+        return np.array(out, np.int32)
+
+    t0 = time.perf_counter()
+    npairs, _ = O.ref32_scan_tiles(wl.words, wl.nid, wl.y[0], tiles(threads), T, wl.test, wl.t1, wl.t2)
+    dt = time.perf_counter() - t0
+    k = max(threads, int(threads * max(1.0, seconds / max(dt, 1e-3))))
+    tl = tiles(k)
+    t0 = time.perf_counter()
+    npairs, nh = O.ref32_scan_tiles(wl.words, wl.nid, wl.y[0], tl, T, wl.test, wl.t1, wl.t2)
+    dt = time.perf_counter() - t0
+    return {"value": npairs / dt, "unit": "pair-tests/s", "cores": threads, "kind": "port",
+            "sample": f"{len(tl)} seeded 128x128 off-diagonal tiles of the workload ({npairs} pairs, {dt:.1f} s)"}
+
+
+def accuracy(s, wl, form):
+    """One 64x64 block of pairs against the FP64 oracle."""
+    import oracle as O
+    a0, b0 = wl.nsnp // 2 + 17, wl.nsnp // 3 + 5
+    counts, sums, F, F2 = s.pair_dump(a0, a0 + 64, b0, b0 + 64, test=wl.test, form=form)
+    codes = O.unpack(wl.words[np.r_[a0:a0 + 64, b0:b0 + 64]], wl.nid)
+    aa, bb = np.meshgrid(np.arange(64), np.arange(64, 128), indexing="ij")
+    pr = np.stack([aa.ravel(), bb.ravel()], 1)
+    ref = O.pairs(codes, wl.y[0], pr)
+    c = counts.reshape(-1, 9)
+    rel = lambda x, r: np.abs(x - r) / np.maximum(np.abs(r), 1.0)
+    return {"pairs_checked": int(len(pr)), "counts_exact": bool((c == ref["counts"]).all()),
+            "f_relerr_max": float(rel(F.ravel(), ref["F"]).max()),
+            "f2_relerr_max": float(rel(F2.ravel(), ref["F2"]).max())}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c3")
+    ap.add_argument("--form", default="mfma", choices=["mfma", "plain"])
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison scan")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-check", action="store_true", help="skip the oracle check")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: the scan runs on one MI355X per process and this build has no "
+              "multi-GPU path (pairs shard as independent tile ranges; see DESIGN.md). Use --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import epigpu_amd as E
+    from epigpu_amd import workloads
+
+    wl = workloads.get(args.config)
+    t0 = time.perf_counter()
+    wl.generate()
+    log(f"[bench] generated {wl.name}: N={wl.nid} M={wl.nsnp} pairs={wl.pairs} in {time.perf_counter() - t0:.1f} s")
+
+    s = E.Scanner(0)
+    log(f"[bench] device {s.device_name()}")
+    t0 = time.perf_counter()
+    s.set_genotypes(wl.words, wl.nid)
+    s.set_phenotypes(wl.y)
+    s.set_chromosomes(wl.chr_starts, wl.chr_sizes)
+    prep_s = time.perf_counter() - t0
+    log(f"[bench] upload + transcode {prep_s * 1e3:.1f} ms")
+
+    def step(form):
+        return s.scan(test=wl.test, form=form, t1=wl.t1, t2=wl.t2)
+
+    for i in range(args.warmup):
+        r = step(args.form)
+        log(f"[bench] warmup {i}: {r['kernel_ms']:.1f} ms kernel, {r['nhits']} hits")
+    kms = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        r = step(args.form)
+        kms.append(r["kernel_ms"])
+        log(f"[bench] step {i}: {r['kernel_ms']:.1f} ms kernel, {r['nhits']} hits")
+    elapsed = time.perf_counter() - t0
+    value = wl.pairs * args.steps / elapsed
+    hits = r["hits"]
+    found = 0
+    if wl.planted_pairs is not None:
+        hs = set(zip(hits["snp1"].tolist(), hits["snp2"].tolist()))
+        found = sum((int(a), int(b)) in hs for a, b, _, _ in wl.planted_pairs)
+
+    kernel_ms = float(np.mean(kms))
+    logical_ops = 36.0 * wl.nid * wl.pairs  # 2*9*N count ops + 2*9*N sum ops per pair
+    achieved = logical_ops / (kernel_ms * 1e-3) / 1e12
+    out = {
+        "metric": METRIC, "value": value, "unit": "pair-tests/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int8" if args.form == "mfma" else "fp32",
+        "data": "synthetic (seeded generator, see epigpu_amd/workloads.py)",
+        "config": {"workload": f"{wl.name}: N={wl.nid}, M={wl.nsnp} ({len(wl.chr_sizes)} chr), "
+                               f"{wl.pairs} pairs, test {wl.test}, t1={wl.t1:g}, t2={wl.t2:g}, "
+                               f"MAF U{wl.maf}, genotype missing {wl.gmiss}",
+                   "form": args.form, "pairs_per_step": wl.pairs},
+        "kernel_ms_per_step": kernel_ms,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
+                     "frac": achieved / INT8_PEAK_TOPS, "traffic": None,
+                     "ops_per_pair": 36 * wl.nid, "note": "INT8 ops; 36*N logical ops per pair"},
+        "hits": int(r["nhits"]), "planted_found": found,
+        "prep_ms": prep_s * 1e3,
+    }
+    if not args.no_plain and args.form == "mfma":
+        r2 = step("plain")
+        out["plain_kernel_ms"] = r2["kernel_ms"]
+        out["plain_pairs_per_s"] = wl.pairs / (r2["kernel_ms"] * 1e-3)
+        out["mfma_over_plain"] = out["plain_kernel_ms"] / kernel_ms
+        same = set(zip(r2["hits"]["snp1"].tolist(), r2["hits"]["snp2"].tolist())) == \
+            set(zip(hits["snp1"].tolist(), hits["snp2"].tolist()))
+        out["plain_same_hits"] = bool(same)
+        log(f"[bench] plain form: {r2['kernel_ms']:.1f} ms kernel; mfma/plain = {out['mfma_over_plain']:.2f}x")
+    if not args.no_check:
+        out["check"] = accuracy(s, wl, args.form)
+        log(f"[bench] oracle check: {out['check']}")
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", min(16, os.cpu_count() or 1)))
+        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds, threads)
+        log(f"[bench] cpu baseline: {out['cpu_baseline']}")
+    s.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,151 @@
+/*
+ * episcan - analysis-mode CLI (mirror of src/main.c:58-188, :283-331).
+ *
+ *   episcan -A <data.epigpu> <output.txt> [options]
+ *     -t f|i     8df test, or 8df + 4df interaction test (default f)
+ *     -F x       F threshold of the 8df test (default 6.5)
+ *     -I x       F threshold of the 4df test (default 10.5)
+ *     -f p       set -F from a p-value: F^-1(1-p; 8, N-9) (extension)
+ *     -g p       set -I from a p-value: F^-1(1-p; 4, N-9) (extension)
+ *     -d n       HIP device (default 0)
+ *     -p n       permutation n: shuffle phenotypes with srand(n), as the reference
+ *     -P file    phenotype file, one value per individual
+ *     -m plain|mfma  kernel form (default mfma) (extension)
+ *     -i n, -s n accepted for compatibility and ignored
+ * Options come in pairs, as in the reference (src/main.c:63).
+ */
+#include <ctype.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "episcan.h"
+#include "epihost.h"
+
+static void usage(void)
+{
+    fprintf(stderr,
+            "usage: episcan -A <data.epigpu> <output.txt> [-t f|i] [-F x] [-I x] [-f p] [-g p]\n"
+            "               [-d n] [-p n] [-P file] [-m plain|mfma]\n");
+}
+
+static int all_digits(const char *s)
+{
+    if (!*s) return 0;
+    for (; *s; s++)
+        if (!isdigit((unsigned char)*s)) return 0;
+    return 1;
+}
+
+static const char *base_name(const char *p)
+{
+    const char *b = p;
+    for (; *p; p++)
+        if (*p == '/' || *p == '\\') b = p + 1;
+    return b;
+}
+
+/* Fisher-Yates with the C library rand() as the reference's permute()
+ * (src/main.c:336-357), applied to the phenotype records. */
+static void permute_ped(int nid, ped *dat)
+{
+    for (int i = nid; i > 1; i--) {
+        int r = rand() % i;
+        ped t = dat[r];
+        dat[r] = dat[i - 1];
+        dat[i - 1] = t;
+    }
+}
+
+/* 1/2-coded phenotype check (src/readplink.c:118-147). */
+static int is_binary(int nid, const ped *dat)
+{
+    int c1 = 0, c2 = 0, other = 0;
+    for (int i = 0; i < nid; i++) {
+        if (dat[i].phen == 1.0f) c1++;
+        else if (dat[i].phen == 2.0f) c2++;
+        else other++;
+    }
+    return other == 0 && c1 + c2 == nid;
+}
+
+int main(int argc, char **argv)
+{
+    if (argc < 4 || strcmp(argv[1], "-A") != 0 || (argc & 1) != 0) {
+        usage();
+        return 1;
+    }
+    const char *bin = argv[2], *outfile = argv[3];
+    epi_run_opts o;
+    epi_run_opts_default(&o);
+    const char *phenfile = "";
+    double pf = -1.0, pi = -1.0;
+    for (int i = 4; i < argc; i += 2) {
+        const char *f = argv[i], *v = argv[i + 1];
+        if (f[0] != '-' || !f[1] || f[2]) { usage(); return 1; }
+        switch (f[1]) {
+        case 't':
+            if (v[0] != 'f' && v[0] != 'i') { fprintf(stderr, "Test selection: f or i\n"); return 1; }
+            o.test = v[0];
+            break;
+        case 'F': o.t1 = (float)atof(v); if (o.t1 == 0.0f) { fprintf(stderr, "numeric -F required\n"); return 1; } break;
+        case 'I': o.t2 = (float)atof(v); if (o.t2 == 0.0f) { fprintf(stderr, "numeric -I required\n"); return 1; } break;
+        case 'f': pf = atof(v); break;
+        case 'g': pi = atof(v); break;
+        case 'd': if (!all_digits(v)) { fprintf(stderr, "numeric -d required\n"); return 1; } o.device = atoi(v); break;
+        case 'p': if (!all_digits(v)) { fprintf(stderr, "numeric -p required\n"); return 1; } o.perm = atoi(v); break;
+        case 'P': phenfile = v; break;
+        case 'm':
+            if (strcmp(v, "plain") == 0) o.form = EPI_FORM_PLAIN;
+            else if (strcmp(v, "mfma") == 0) o.form = EPI_FORM_MFMA;
+            else { fprintf(stderr, "-m plain|mfma\n"); return 1; }
+            break;
+        case 'i': {
+            int k = atoi(v);
+            if (!all_digits(v) || k < 16 || k % 16) { fprintf(stderr, "Iteration size must be a multiple of 16\n"); return 1; }
+            break;
+        }
+        case 's': if (strcmp(v, "0") && strcmp(v, "1")) { fprintf(stderr, "-s 0|1\n"); return 1; } break;
+        default: usage(); return 1;
+        }
+    }
+    epi_container c;
+    int rc = epi_read_container(bin, &c);
+    if (rc) {
+        fprintf(stderr, "%s: %s\n", bin, epih_strerror(rc));
+        return 1;
+    }
+    if (phenfile[0]) {
+        FILE *pfh = fopen(phenfile, "rt");
+        if (!pfh) { fprintf(stderr, "Separate phenotype file does not exist\n"); return 1; }
+        for (int i = 0; i < c.nid; i++) {
+            float t;
+            if (fscanf(pfh, "%f", &t) != 1) { fprintf(stderr, "phenotype file too short\n"); return 1; }
+            c.dat[i].phen = t;
+        }
+        fclose(pfh);
+    }
+    if (is_binary(c.nid, c.dat)) {
+        fprintf(stderr, "Error: Binary phenotype!\nepiGPU can only use quantitative phenotypes for analysis.\n");
+        return 1;
+    }
+    if (o.perm > 0) {
+        srand((unsigned)o.perm);
+        permute_ped(c.nid, c.dat);
+    }
+    if (pf > 0.0) o.t1 = (float)epi_f_isf(pf, 8.0, (double)c.nid - 9.0);
+    if (pi > 0.0) o.t2 = (float)epi_f_isf(pi, 4.0, (double)c.nid - 9.0);
+    o.datafile = base_name(bin);
+    o.phenfile = phenfile;
+    epi_run_report rep;
+    rc = epi_perform_scan(&o, c.geno, c.chrstat, c.dat, c.genmap, c.nid, c.nsnp, c.npack, c.remain, c.nchr, outfile,
+                          &rep);
+    epi_container_free(&c);
+    if (rc) {
+        fprintf(stderr, "episcan: %s (%d)\n", epih_strerror(rc), rc);
+        return 1;
+    }
+    printf("pairs %lld  device %.3f s  wall %.3f s  hits %lld\n", (long long)rep.pairs, rep.kernel_s, rep.wall_s,
+           (long long)rep.total_hits);
+    return 0;
+}

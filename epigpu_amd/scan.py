@@ -1,0 +1,404 @@
+"""ctypes bindings of the native libraries (plumbing for tests and the benchmark).
+
+The product is libepiscan.so (episcan.h) and libepihost.so (epihost.h); this
+module only moves numpy arrays across their C ABI. Loading fails loudly when a
+library has not been built: there is no Python fallback for any computation.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+LIBDIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
+
+FORM_PLAIN = 0
+FORM_MFMA = 1
+FORMS = {"plain": FORM_PLAIN, "mfma": FORM_MFMA}
+
+PHEN_NORMAL, PHEN_GAMMA2, PHEN_GAMMA2_STD, PHEN_T5, PHEN_BERN02 = range(5)
+
+
+class EpiError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{msg} (code {code})")
+        self.code = code
+
+
+class Hit(ctypes.Structure):
+    """Reference `table` record (src/id4117.h:14-21)."""
+    _fields_ = [("snp1", ctypes.c_int32), ("snp2", ctypes.c_int32), ("df1", ctypes.c_int32),
+                ("df2", ctypes.c_int32), ("F", ctypes.c_float), ("F2", ctypes.c_float)]
+
+
+HIT_DTYPE = np.dtype([("snp1", "<i4"), ("snp2", "<i4"), ("df1", "<i4"), ("df2", "<i4"),
+                      ("F", "<f4"), ("F2", "<f4")])
+
+
+class ScanParams(ctypes.Structure):
+    _fields_ = [("test", ctypes.c_int32), ("form", ctypes.c_int32), ("t1", ctypes.c_float),
+                ("t2", ctypes.c_float), ("phen", ctypes.c_int32), ("blk_begin", ctypes.c_int32),
+                ("blk_end", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4)]
+
+
+class RunOpts(ctypes.Structure):
+    _fields_ = [("test", ctypes.c_int), ("t1", ctypes.c_float), ("t2", ctypes.c_float),
+                ("device", ctypes.c_int), ("form", ctypes.c_int), ("perm", ctypes.c_int),
+                ("datafile", ctypes.c_char_p), ("phenfile", ctypes.c_char_p), ("quiet", ctypes.c_int)]
+
+
+class RunReport(ctypes.Structure):
+    _fields_ = [("total_hits", ctypes.c_int64), ("new_hits", ctypes.c_int64), ("blocks_done", ctypes.c_int),
+                ("first_block", ctypes.c_int), ("overflow", ctypes.c_int), ("kernel_s", ctypes.c_double),
+                ("wall_s", ctypes.c_double), ("pairs", ctypes.c_int64)]
+
+
+class ResumeState(ctypes.Structure):
+    _fields_ = [("exists", ctypes.c_int), ("chr1", ctypes.c_int), ("chr2", ctypes.c_int),
+                ("tothits", ctypes.c_int64), ("blocks", ctypes.c_int)]
+
+
+class Container(ctypes.Structure):
+    _fields_ = [("nid", ctypes.c_int), ("nsnp", ctypes.c_int), ("npack", ctypes.c_int), ("remain", ctypes.c_int),
+                ("nchr", ctypes.c_int), ("dat", ctypes.c_void_p), ("genmap", ctypes.c_void_p),
+                ("chrstat", ctypes.c_void_p), ("geno", ctypes.c_void_p)]
+
+
+# Record layouts of the epiGPU container (src/readplink.h:17-39, x86-64 gcc).
+PED_DTYPE = np.dtype({"names": ["family", "id", "paternal", "maternal", "sex", "phen"],
+                      "formats": ["S30", "S30", "S30", "S30", "S1", "<f4"],
+                      "offsets": [0, 30, 60, 90, 120, 124], "itemsize": 128})
+MAP_DTYPE = np.dtype({"names": ["chr", "snpname", "gd", "pd", "allelename"],
+                      "formats": ["S30", "S30", "<f8", "<i8", "S2"],
+                      "offsets": [0, 30, 64, 72, 80], "itemsize": 88})
+CHR_DTYPE = np.dtype({"names": ["chrid", "chrname", "chrsize", "chrstart"],
+                      "formats": ["<i4", "S30", "<i4", "<i4"],
+                      "offsets": [0, 4, 36, 40], "itemsize": 44})
+
+_libs = {}
+
+
+def _load(name):
+    if name not in _libs:
+        path = os.path.join(LIBDIR, name)
+        if not os.path.exists(path):
+            raise ImportError(f"{path} is not built; run __graft_entry__.build()")
+        _libs[name] = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+        _declare(name, _libs[name])
+    return _libs[name]
+
+
+def _p(t):
+    return ctypes.POINTER(t)
+
+
+def _declare(name, L):
+    i32, i64, f32, f64, vp = ctypes.c_int32, ctypes.c_int64, ctypes.c_float, ctypes.c_double, ctypes.c_void_p
+    if name == "libepiscan.so":
+        sig = {
+            "epi_open": (ctypes.c_int, [ctypes.c_int, _p(vp)]),
+            "epi_close": (None, [vp]),
+            "epi_strerror": (ctypes.c_char_p, [ctypes.c_int]),
+            "epi_last_error": (ctypes.c_char_p, [vp]),
+            "epi_device_name": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_size_t]),
+            "epi_set_genotypes": (ctypes.c_int, [vp, vp, i32, i32]),
+            "epi_set_phenotypes": (ctypes.c_int, [vp, vp, i32, i32]),
+            "epi_set_chromosomes": (ctypes.c_int, [vp, vp, vp, i32]),
+            "epi_scan_params_default": (None, [_p(ScanParams)]),
+            "epi_scan": (ctypes.c_int, [vp, _p(ScanParams)]),
+            "epi_hit_count": (ctypes.c_int, [vp, _p(i64), _p(i32)]),
+            "epi_get_hits": (ctypes.c_int, [vp, vp, vp, vp, vp, i64, _p(i64)]),
+            "epi_set_hit_capacity": (ctypes.c_int, [vp, i64]),
+            "epi_last_scan_stats": (ctypes.c_int, [vp, _p(f64), _p(i64), _p(i32)]),
+            "epi_pair_dump": (ctypes.c_int, [vp, _p(ScanParams), i32, i32, i32, i32, vp, vp, vp, vp]),
+        }
+    else:
+        sig = {
+            "epi_layout": (None, [ctypes.c_int, _p(ctypes.c_int), _p(ctypes.c_int), _p(ctypes.c_int)]),
+            "epi_pack": (None, [vp, ctypes.c_int, ctypes.c_int, vp]),
+            "epi_unpack": (None, [vp, ctypes.c_int, ctypes.c_int, vp]),
+            "epi_synth_genotypes": (None, [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, f64, f64, f64, vp]),
+            "epi_synth_phenotype": (None, [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, f64, vp]),
+            "epi_synth_plant": (None, [ctypes.c_uint64, vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, f32, vp]),
+            "epi_betai": (f64, [f64, f64, f64]),
+            "epi_f_sf": (f64, [f64, f64, f64]),
+            "epi_f_isf": (f64, [f64, f64, f64]),
+            "epi_read_container": (ctypes.c_int, [ctypes.c_char_p, _p(Container)]),
+            "epi_write_container": (ctypes.c_int, [ctypes.c_char_p, _p(Container)]),
+            "epi_container_free": (None, [_p(Container)]),
+            "epih_strerror": (ctypes.c_char_p, [ctypes.c_int]),
+            "epi_run_opts_default": (None, [_p(RunOpts)]),
+            "epi_perform_scan": (ctypes.c_int, [_p(RunOpts), vp, vp, vp, vp, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
+                                                _p(RunReport)]),
+            "epi_resume": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_int, f32, f32, _p(ResumeState)]),
+            "epi_format_row": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p,
+                                              ctypes.c_char_p, ctypes.c_int, ctypes.c_int, f32, f32]),
+        }
+    for fn, (res, args) in sig.items():
+        f = getattr(L, fn)
+        f.restype = res
+        f.argtypes = args
+
+
+def episcan():
+    return _load("libepiscan.so")
+
+
+def epihost():
+    episcan()
+    return _load("libepihost.so")
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+# ------------------------------------------------------------- host side --
+
+def layout(nid):
+    npack, remain = nid // 16, nid % 16
+    return npack, remain, npack + (remain > 0)
+
+
+def pack(codes):
+    """codes: (nsnp, nid) int8 in {0,1,2,3} -> (nsnp, npack1) int32 words (layout A1)."""
+    codes = np.ascontiguousarray(codes, dtype=np.int8)
+    nsnp, nid = codes.shape
+    words = np.empty((nsnp, layout(nid)[2]), dtype=np.int32)
+    epihost().epi_pack(_ptr(codes), nid, nsnp, _ptr(words))
+    return words
+
+
+def unpack(words, nid):
+    words = np.ascontiguousarray(words, dtype=np.int32)
+    nsnp = words.shape[0]
+    codes = np.empty((nsnp, nid), dtype=np.int8)
+    epihost().epi_unpack(_ptr(words), nid, nsnp, _ptr(codes))
+    return codes
+
+
+def synth_genotypes(seed, nid, nsnp, maf_lo=0.05, maf_hi=0.5, miss=0.0):
+    words = np.empty((nsnp, layout(nid)[2]), dtype=np.int32)
+    epihost().epi_synth_genotypes(seed, nid, nsnp, maf_lo, maf_hi, miss, _ptr(words))
+    return words
+
+
+def synth_phenotype(seed, r, nid, dist=PHEN_NORMAL, miss=0.0):
+    y = np.empty(nid, dtype=np.float32)
+    epihost().epi_synth_phenotype(seed, r, nid, dist, miss, _ptr(y))
+    return y
+
+
+def synth_plant(seed, words, nid, phen, npairs, effect):
+    words = np.ascontiguousarray(words, dtype=np.int32)
+    pairs = np.zeros((npairs, 4), dtype=np.int32)
+    epihost().epi_synth_plant(seed, _ptr(words), nid, words.shape[0], _ptr(phen), npairs, effect, _ptr(pairs))
+    return pairs
+
+
+def f_sf(F, d1, d2):
+    return epihost().epi_f_sf(F, d1, d2)
+
+
+def f_isf(p, d1, d2):
+    return epihost().epi_f_isf(p, d1, d2)
+
+
+def betai(a, b, x):
+    return epihost().epi_betai(a, b, x)
+
+
+def read_container(path):
+    c = Container()
+    rc = epihost().epi_read_container(os.fsencode(path), ctypes.byref(c))
+    if rc:
+        raise EpiError(rc, epihost().epih_strerror(rc).decode())
+    try:
+        npack1 = c.npack + (c.remain > 0)
+        def arr(ptr, dtype, n):
+            buf = (ctypes.c_char * (dtype.itemsize * n)).from_address(ptr)
+            return np.frombuffer(bytearray(buf), dtype=dtype)
+        out = {
+            "nid": c.nid, "nsnp": c.nsnp, "npack": c.npack, "remain": c.remain, "nchr": c.nchr,
+            "ped": arr(c.dat, PED_DTYPE, c.nid),
+            "map": arr(c.genmap, MAP_DTYPE, c.nsnp),
+            "chr": arr(c.chrstat, CHR_DTYPE, c.nchr),
+            "geno": arr(c.geno, np.dtype("<i4"), npack1 * c.nsnp).reshape(c.nsnp, npack1),
+        }
+    finally:
+        epihost().epi_container_free(ctypes.byref(c))
+    return out
+
+
+def make_records(nid, nsnp, chr_sizes, phen):
+    """ped/map/chromosome records for a synthetic dataset."""
+    ped = np.zeros(nid, dtype=PED_DTYPE)
+    for i in range(nid):
+        ped[i]["family"] = b"F%d" % i
+        ped[i]["id"] = b"I%d" % i
+    ped["sex"] = b"1"
+    ped["phen"] = phen
+    mp = np.zeros(nsnp, dtype=MAP_DTYPE)
+    chrs = np.zeros(len(chr_sizes), dtype=CHR_DTYPE)
+    start = 0
+    for i, sz in enumerate(chr_sizes):
+        chrs[i]["chrid"] = i + 1
+        chrs[i]["chrname"] = b"%d" % (i + 1)
+        chrs[i]["chrsize"] = sz
+        chrs[i]["chrstart"] = start
+        for s in range(start, start + sz):
+            mp[s]["chr"] = b"%d" % (i + 1)
+            mp[s]["snpname"] = b"rs%d" % (s + 1)
+            mp[s]["pd"] = s - start + 1
+        start += sz
+    return ped, mp, chrs
+
+
+def write_container(path, words, nid, ped, mp, chrs):
+    words = np.ascontiguousarray(words, dtype=np.int32)
+    npack, remain, _ = layout(nid)
+    ped = np.ascontiguousarray(ped)
+    mp = np.ascontiguousarray(mp)
+    chrs = np.ascontiguousarray(chrs)
+    c = Container(nid, words.shape[0], npack, remain, len(chrs), ped.ctypes.data, mp.ctypes.data,
+                  chrs.ctypes.data, words.ctypes.data)
+    rc = epihost().epi_write_container(os.fsencode(path), ctypes.byref(c))
+    if rc:
+        raise EpiError(rc, epihost().epih_strerror(rc).decode())
+
+
+def perform_scan(words, nid, ped, mp, chrs, filename, test="f", t1=6.5, t2=10.5, form="mfma", device=0,
+                 datafile="data.epigpu", phenfile="", perm=0, quiet=True):
+    """Run the C host driver (mirror of the reference's perform_scan)."""
+    H = epihost()
+    o = RunOpts()
+    H.epi_run_opts_default(ctypes.byref(o))
+    o.test, o.t1, o.t2, o.device, o.form, o.perm = ord(test), t1, t2, device, FORMS[form], perm
+    o.datafile, o.phenfile, o.quiet = datafile.encode(), phenfile.encode(), int(quiet)
+    rep = RunReport()
+    words = np.ascontiguousarray(words, dtype=np.int32)
+    npack, remain, _ = layout(nid)
+    ped, mp, chrs = (np.ascontiguousarray(x) for x in (ped, mp, chrs))
+    rc = H.epi_perform_scan(ctypes.byref(o), _ptr(words), _ptr(chrs), _ptr(ped), _ptr(mp), nid, words.shape[0],
+                            npack, remain, len(chrs), os.fsencode(filename), ctypes.byref(rep))
+    if rc:
+        raise EpiError(rc, H.epih_strerror(rc).decode())
+    return {f: getattr(rep, f) for f, _ in RunReport._fields_}
+
+
+def resume_state(filename, datafile, nid, nsnp, perm=0, test="f", t1=6.5, t2=10.5):
+    st = ResumeState()
+    rc = epihost().epi_resume(os.fsencode(filename), datafile.encode(), nid, nsnp, perm, ord(test), t1, t2,
+                              ctypes.byref(st))
+    if rc:
+        raise EpiError(rc, epihost().epih_strerror(rc).decode())
+    return {f: getattr(st, f) for f, _ in ResumeState._fields_}
+
+
+def format_row(test, snp1, name1, chr1, snp2, name2, chr2, df1, df2, F, F2=0.0):
+    buf = ctypes.create_string_buffer(512)
+    epihost().epi_format_row(buf, 512, ord(test), snp1, name1.encode(), chr1.encode(), snp2, name2.encode(),
+                             chr2.encode(), df1, df2, F, F2)
+    return buf.value.decode()
+
+
+# ----------------------------------------------------------- device side --
+
+class Scanner:
+    """One episcan.h context (one HIP stream on one device)."""
+
+    def __init__(self, device=0):
+        self.L = episcan()
+        self.ctx = ctypes.c_void_p()
+        rc = self.L.epi_open(device, ctypes.byref(self.ctx))
+        if rc:
+            raise EpiError(rc, "epi_open: " + self.L.epi_strerror(rc).decode())
+        self.nid = None
+
+    def _chk(self, rc, what):
+        if rc:
+            raise EpiError(rc, f"{what}: {self.L.epi_last_error(self.ctx).decode()}")
+
+    def close(self):
+        if self.ctx:
+            self.L.epi_close(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_name(self):
+        buf = ctypes.create_string_buffer(256)
+        self.L.epi_device_name(self.ctx, buf, 256)
+        return buf.value.decode()
+
+    def set_genotypes(self, words, nid):
+        words = np.ascontiguousarray(words, dtype=np.int32)
+        assert words.shape[1] == layout(nid)[2]
+        self._chk(self.L.epi_set_genotypes(self.ctx, _ptr(words), nid, words.shape[0]), "epi_set_genotypes")
+        self.nid = nid
+
+    def set_phenotypes(self, phen):
+        phen = np.ascontiguousarray(np.atleast_2d(phen), dtype=np.float32)
+        self._chk(self.L.epi_set_phenotypes(self.ctx, _ptr(phen), phen.shape[0], phen.shape[1]),
+                  "epi_set_phenotypes")
+
+    def set_chromosomes(self, starts, sizes):
+        s = np.ascontiguousarray(starts, dtype=np.int32)
+        z = np.ascontiguousarray(sizes, dtype=np.int32)
+        self._chk(self.L.epi_set_chromosomes(self.ctx, _ptr(s), _ptr(z), len(s)), "epi_set_chromosomes")
+
+    def set_hit_capacity(self, cap):
+        self._chk(self.L.epi_set_hit_capacity(self.ctx, cap), "epi_set_hit_capacity")
+
+    @staticmethod
+    def params(test="f", form="mfma", t1=6.5, t2=10.5, phen=0, blk_begin=0, blk_end=0):
+        p = ScanParams()
+        p.test, p.form, p.t1, p.t2, p.phen = ord(test), FORMS[form], t1, t2, phen
+        p.blk_begin, p.blk_end = blk_begin, blk_end
+        return p
+
+    def scan(self, test="f", form="mfma", t1=6.5, t2=10.5, phen=0, blk_begin=0, blk_end=0, fetch=True):
+        p = self.params(test, form, t1, t2, phen, blk_begin, blk_end)
+        self._chk(self.L.epi_scan(self.ctx, ctypes.byref(p)), "epi_scan")
+        n = ctypes.c_int64()
+        ovf = ctypes.c_int32()
+        self._chk(self.L.epi_hit_count(self.ctx, ctypes.byref(n), ctypes.byref(ovf)), "epi_hit_count")
+        ms, pairs, nl = ctypes.c_double(), ctypes.c_int64(), ctypes.c_int32()
+        self.L.epi_last_scan_stats(self.ctx, ctypes.byref(ms), ctypes.byref(pairs), ctypes.byref(nl))
+        out = {"nhits": n.value, "overflow": bool(ovf.value), "kernel_ms": ms.value, "pairs": pairs.value,
+               "launches": nl.value}
+        if fetch:
+            m = min(n.value, 1 << 40)
+            hits = np.zeros(m, dtype=HIT_DTYPE)
+            F = np.zeros(m, dtype=np.float64)
+            F2 = np.zeros(m, dtype=np.float64)
+            blk = np.zeros(m, dtype=np.int32)
+            got = ctypes.c_int64()
+            self._chk(self.L.epi_get_hits(self.ctx, _ptr(hits), _ptr(F), _ptr(F2), _ptr(blk), m,
+                                          ctypes.byref(got)), "epi_get_hits")
+            g = got.value
+            out.update(hits=hits[:g], F64=F[:g], F2_64=F2[:g], block=blk[:g])
+        return out
+
+    def pair_dump(self, a0, a1, b0, b1, test="i", form="mfma", phen=0):
+        p = self.params(test, form, 6.5, 10.5, phen)
+        na, nb = a1 - a0, b1 - b0
+        counts = np.full((na, nb, 9), -1, dtype=np.int32)
+        sums = np.full((na, nb, 9), np.nan, dtype=np.float64)
+        F = np.full((na, nb), np.nan, dtype=np.float64)
+        F2 = np.full((na, nb), np.nan, dtype=np.float64)
+        self._chk(self.L.epi_pair_dump(self.ctx, ctypes.byref(p), a0, a1, b0, b1, _ptr(counts), _ptr(sums),
+                                       _ptr(F), _ptr(F2)), "epi_pair_dump")
+        return counts, sums, F, F2
