@@ -192,10 +192,13 @@ __device__ __forceinline__ void append_hit(const HitSink &hs, bool hit, const De
 
 // Finish one pair: counts n[9] (exact), sums s[9] of type S (exact integers
 // for the MFMA form, FP64 for the plain form), SQ in the same units squared.
-// valid = the pair is inside the tile and a > b. All lanes must call.
+// Cells are indexed 3*pa + pb by plane; clsa/clsb map plane -> genotype class
+// (2 bits per plane). The statistics do not depend on the class labels; only
+// the dump writes cells back in genotype order. valid = the pair is inside the
+// tile and a > b. All lanes must call.
 template <bool DUMP, class S>
 __device__ __forceinline__ void finish_pair(const HitSink &hs, const DumpSink &ds, bool valid, int a, int b,
-                                            const int n[9], const S s[9], double sq)
+                                            const int n[9], const S s[9], double sq, int clsa, int clsb)
 {
     if (DUMP) {
         if (valid && a >= ds.a0 && a < ds.a1 && b >= ds.b0 && b < ds.b1) {
@@ -204,8 +207,9 @@ __device__ __forceinline__ void finish_pair(const HitSink &hs, const DumpSink &d
             stats64(n, s, sq, F, F2, d1, d2, nfac);
             long long p = (long long)(a - ds.a0) * (ds.b1 - ds.b0) + (b - ds.b0);
             for (int c = 0; c < 9; c++) {
-                if (ds.counts) ds.counts[9 * p + c] = n[c];
-                if (ds.sums) ds.sums[9 * p + c] = (double)s[c] * ds.unit + (double)n[c] * ds.centre;
+                int o = 3 * ((clsa >> (2 * (c / 3))) & 3) + ((clsb >> (2 * (c % 3))) & 3);
+                if (ds.counts) ds.counts[9 * p + o] = n[c];
+                if (ds.sums) ds.sums[9 * p + o] = (double)s[c] * ds.unit + (double)n[c] * ds.centre;
             }
             if (ds.F) ds.F[p] = F;
             if (ds.F2) ds.F2[p] = F2;

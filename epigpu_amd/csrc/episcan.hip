@@ -62,6 +62,7 @@ struct epi_ctx {
     uint8_t *planes = nullptr;
     uint32_t *bits = nullptr;
     int32_t *dmiss = nullptr;
+    int32_t *cls = nullptr;  // per SNP plane -> class map of the plain form
     std::vector<int> miss;   // per SNP: any missing call
     std::vector<int> missc;  // prefix count of SNPs with missing calls
     // phenotypes
@@ -174,19 +175,39 @@ __global__ void transcode_planes(const uint32_t *words, int nid, int npack, int 
     if (anymiss) atomicOr(&miss[s], 1);
 }
 
+// One thread per SNP: order the genotype classes by frequency. The plain form
+// stores the two rarest classes as bitplanes 0 and 1 and derives the most
+// common one; cls packs plane -> class (2 bits each).
+__global__ void class_order(const uint32_t *words, int nid, int npack, int remain, int npack1, int nsnp,
+                            int32_t *cls)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsnp) return;
+    const uint32_t *w = words + (size_t)s * npack1;
+    int n[4] = {0, 0, 0, 0};
+    for (int k = 0; k < nid; k++) n[decode(w, npack, remain, nid, k)]++;
+    int big = 0;
+    if (n[1] > n[big]) big = 1;
+    if (n[2] > n[big]) big = 2;
+    int p0 = big == 0 ? 1 : 0;
+    int p1 = big == 2 ? 1 : 2;
+    cls[s] = p0 | (p1 << 2) | (big << 4);
+}
+
 // One thread per (word of 32 individuals, snp): class bitplanes, plain form.
 __global__ void transcode_bits(const uint32_t *words, int nid, int npack, int remain, int npack1, int nsnp,
-                               int Kw, uint32_t *bits)
+                               int Kw, const int32_t *cls, uint32_t *bits)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     int wd = blockIdx.y;
     if (s >= nsnp) return;
     const uint32_t *w = words + (size_t)s * npack1;
+    const int c0 = cls[s] & 3, c1 = (cls[s] >> 2) & 3;
     uint32_t b0 = 0, b1 = 0, bn = 0;
     for (int t = 0; t < 32; t++) {
         int c = decode(w, npack, remain, nid, 32 * wd + t);
-        b0 |= (uint32_t)(c == 0) << t;
-        b1 |= (uint32_t)(c == 1) << t;
+        b0 |= (uint32_t)(c == c0) << t;
+        b1 |= (uint32_t)(c == c1) << t;
         bn |= (uint32_t)(c >= 0 && c != 3) << t;
     }
     bits[((size_t)0 * Kw + wd) * nsnp + s] = b0;
@@ -194,11 +215,12 @@ __global__ void transcode_bits(const uint32_t *words, int nid, int npack, int re
     bits[((size_t)2 * Kw + wd) * nsnp + s] = bn;
 }
 
-// One workgroup per SNP: class counts and phenotype sums over present
-// phenotypes, for one phenotype.
+// One workgroup per SNP: per genotype class (0, 1, 2) counts and phenotype
+// sums over present phenotypes, for one phenotype; stored per class for the
+// MFMA form and per bitplane (cls order) for the plain form.
 __global__ __launch_bounds__(256) void marginals_kernel(const uint8_t *planes, int Kp, int nsnp,
                                                         const int8_t *wts, const int32_t *qv,
-                                                        const float *yc, Marg *marg)
+                                                        const float *yc, const int32_t *cls, Marg *marg)
 {
     __shared__ long long sq[3][256];
     __shared__ double sy[3][256];
@@ -210,13 +232,11 @@ __global__ __launch_bounds__(256) void marginals_kernel(const uint8_t *planes, i
     const uint8_t *pl = planes + (size_t)s * 3 * Kp;
     for (int k = tid; k < Kp; k += 256) {
         if (!wts[k]) continue;  // row 0 = V
-        for (int p = 0; p < 3; p++) {
-            if (pl[(size_t)p * Kp + k]) {
-                n[p] += 1;
-                q[p] += qv[k];
-                y[p] += (double)yc[k];
-            }
-        }
+        if (!pl[(size_t)2 * Kp + k]) continue;  // missing call
+        int c = pl[k] ? 0 : (pl[(size_t)Kp + k] ? 1 : 2);
+        n[c] += 1;
+        q[c] += qv[k];
+        y[c] += (double)yc[k];
     }
     for (int p = 0; p < 3; p++) {
         sn[p][tid] = n[p];
@@ -235,11 +255,22 @@ __global__ __launch_bounds__(256) void marginals_kernel(const uint8_t *planes, i
     }
     if (tid == 0) {
         Marg m;
-        for (int p = 0; p < 3; p++) {
-            m.n[p] = sn[p][0];
-            m.q[p] = sq[p][0];
-            m.ys[p] = sy[p][0];
+        const int cl = cls[s];
+        const int ntot = sn[0][0] + sn[1][0] + sn[2][0];
+        m.n[0] = sn[0][0];
+        m.n[1] = sn[1][0];
+        m.n[2] = ntot;
+        m.q[0] = sq[0][0];
+        m.q[1] = sq[1][0];
+        m.q[2] = sq[0][0] + sq[1][0] + sq[2][0];
+        for (int p = 0; p < 2; p++) {
+            int c = (cl >> (2 * p)) & 3;
+            m.pn[p] = sn[c][0];
+            m.ys[p] = sy[c][0];
         }
+        m.pn[2] = ntot;
+        m.ys[2] = sy[0][0] + sy[1][0] + sy[2][0];
+        m.cls = cl;
         m.pad = 0;
         marg[s] = m;
     }
@@ -256,7 +287,7 @@ int ensure_marginals(epi_ctx *c)
         int rc = dalloc(c, p.marg, (size_t)c->nsnp);
         if (rc) return rc;
         hipLaunchKernelGGL(marginals_kernel, dim3(c->nsnp), dim3(256), 0, c->stream, c->planes, c->Kp,
-                           c->nsnp, p.wts, p.qv, p.yc, p.marg);
+                           c->nsnp, p.wts, p.qv, p.yc, c->cls, p.marg);
         HIPCHK(c, hipGetLastError());
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -491,6 +522,7 @@ void epi_close(epi_ctx *c)
     dfree(c->planes);
     dfree(c->bits);
     dfree(c->dmiss);
+    dfree(c->cls);
     for (auto &p : c->ph) free_phen(p);
     dfree(c->hits);
     dfree(c->nhits);
@@ -513,7 +545,11 @@ int epi_set_genotypes(epi_ctx *c, const int32_t *words, int32_t nid, int32_t nsn
     if (!c) return EPI_ERR_ARG;
     if (!words || nid < 2 || nsnp < 2) return fail(c, EPI_ERR_ARG, "need nid >= 2, nsnp >= 2 and words");
     HIPCHK(c, hipSetDevice(c->device));
-    if (c->nphen > 0 && nid != c->nid) return fail(c, EPI_ERR_ARG, "nid differs from the phenotypes' nid");
+    if (c->nphen > 0 && nid != c->nid) {  // a new cohort: phenotypes of the old one no longer apply
+        for (auto &p : c->ph) free_phen(p);
+        c->ph.clear();
+        c->nphen = 0;
+    }
     c->nid = nid;
     c->nsnp = nsnp;
     c->npack = nid / 16;
@@ -524,7 +560,8 @@ int epi_set_genotypes(epi_ctx *c, const int32_t *words, int32_t nid, int32_t nsn
     size_t nwords = (size_t)nsnp * c->npack1;
     int rc;
     if ((rc = dalloc(c, c->words, nwords)) || (rc = dalloc(c, c->planes, (size_t)nsnp * 3 * c->Kp)) ||
-        (rc = dalloc(c, c->bits, (size_t)3 * c->Kw * nsnp)) || (rc = dalloc(c, c->dmiss, (size_t)nsnp)))
+        (rc = dalloc(c, c->bits, (size_t)3 * c->Kw * nsnp)) || (rc = dalloc(c, c->dmiss, (size_t)nsnp)) ||
+        (rc = dalloc(c, c->cls, (size_t)nsnp)))
         return rc;
     HIPCHK(c, hipMemcpyAsync(c->words, words, nwords * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemsetAsync(c->dmiss, 0, (size_t)nsnp * sizeof(int32_t), c->stream));
@@ -534,8 +571,12 @@ int epi_set_genotypes(epi_ctx *c, const int32_t *words, int32_t nid, int32_t nsn
         hipLaunchKernelGGL(transcode_planes, dim3(blocks), dim3(256), 0, c->stream, (const uint32_t *)c->words, nid,
                            c->npack, c->remain, c->npack1, nsnp, c->Kp, c->planes, c->dmiss);
         HIPCHK(c, hipGetLastError());
+        hipLaunchKernelGGL(class_order, dim3((nsnp + 255) / 256), dim3(256), 0, c->stream,
+                           (const uint32_t *)c->words, nid, c->npack, c->remain, c->npack1, nsnp, c->cls);
+        HIPCHK(c, hipGetLastError());
         hipLaunchKernelGGL(transcode_bits, dim3((nsnp + 255) / 256, c->Kw), dim3(256), 0, c->stream,
-                           (const uint32_t *)c->words, nid, c->npack, c->remain, c->npack1, nsnp, c->Kw, c->bits);
+                           (const uint32_t *)c->words, nid, c->npack, c->remain, c->npack1, nsnp, c->Kw, c->cls,
+                           c->bits);
         HIPCHK(c, hipGetLastError());
     }
     c->miss.assign(nsnp, 0);
@@ -545,12 +586,11 @@ int epi_set_genotypes(epi_ctx *c, const int32_t *words, int32_t nid, int32_t nsn
     c->missc.assign(nsnp + 1, 0);
     for (int s = 0; s < nsnp; s++) c->missc[s + 1] = c->missc[s] + (c->miss[s] != 0);
     c->marg_ok = false;
-    // default: one chromosome over all SNPs
-    if (c->chrstart.empty() || c->snpchr.size() != (size_t)nsnp) {
-        c->chrstart.assign(1, 0);
-        c->chrsize.assign(1, nsnp);
-        c->snpchr.assign(nsnp, 0);
-    }
+    // chromosome layout resets to one block over all SNPs
+    c->chrstart.assign(1, 0);
+    c->chrsize.assign(1, nsnp);
+    c->snpchr.assign(nsnp, 0);
+    c->last_nhits = 0;
     return EPI_OK;
 }
 

@@ -36,11 +36,16 @@ constexpr int KC = 128;          // individuals per LDS stage
 constexpr int ROWB = KC + 16;    // LDS row pitch in bytes (bank-conflict padding)
 constexpr int NW = 7;            // weight rows: V, L0, L1, L2, Z0, Z1, Z2
 
+// Identity class map: plane p holds class p (the MFMA form keeps genotype order).
+constexpr int CLS_IDENTITY = 0 | (1 << 2) | (2 << 4);
+
 struct Marg {                    // per SNP, per phenotype, over present phenotypes
     int32_t n[3];                // count of class 0, 1, NM
+    int32_t cls;                 // plain-form plane -> class map (2 bits per plane)
+    long long q[3];              // sum of fixed-point phenotype of class 0, 1, NM
+    double ys[3];                // plain form: sum of FP32 centred phenotype of plane 0, 1, NM
+    int32_t pn[3];               // plain form: count of plane 0, 1, NM
     int32_t pad;
-    long long q[3];              // sum of fixed-point phenotype per class
-    double ys[3];                // sum of FP32 centred phenotype per class (plain form)
 };
 
 struct MfmaArgs {
@@ -188,7 +193,7 @@ __global__ __launch_bounds__(256, 1) void mfma_fast_kernel(const MfmaArgs A)
                 cells_from_x(xc, n);
                 cells_from_x(xs, s);
                 const bool valid = a < a1 && b < b1 && a > b;
-                finish_pair<DUMP>(A.hs, A.ds, valid, a, b, n, s, A.sqV);
+                finish_pair<DUMP>(A.hs, A.ds, valid, a, b, n, s, A.sqV, CLS_IDENTITY, CLS_IDENTITY);
             }
         }
     }
@@ -310,7 +315,7 @@ __global__ __launch_bounds__(256, 1) void mfma_general_kernel(const MfmaArgs A)
             cells_from_x(xc, n);
             cells_from_x(xs, s);
             const bool valid = a < a1 && b < b1 && a > b;
-            finish_pair<DUMP>(A.hs, A.ds, valid, a, b, n, s, sq);
+            finish_pair<DUMP>(A.hs, A.ds, valid, a, b, n, s, sq, CLS_IDENTITY, CLS_IDENTITY);
         }
     }
 }

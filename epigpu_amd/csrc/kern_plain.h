@@ -7,9 +7,12 @@
 // owns PA x PB pairs:
 //   counts  X[g][h] += popcount(A_g & V & B_h)      (AND + v_bcnt per word)
 //   sums    X[g][h] += y_k * [k in A_g & V] * [k in B_h]   (masked FP32 FMA)
-// FP32 partial sums are promoted to FP64 after every stage. FAST tiles use the
-// classes {0,1}; GENERAL tiles add the NM class and the sum of y^2 over
-// complete cases. The per-pair epilogue is shared with the MFMA form.
+// FP32 partial sums are promoted to FP64 after every word. The two bitplanes
+// of a SNP hold its two rarest genotype classes and the most common class is
+// derived (by subtraction from the NM totals), so a small cell never inherits
+// the rounding error of a large one. FAST tiles use the planes {0,1}; GENERAL
+// tiles add the NM plane and the sum of y^2 over complete cases. The per-pair
+// epilogue is shared with the MFMA form.
 #pragma once
 
 #include "epilogue.h"
@@ -151,22 +154,22 @@ __global__ __launch_bounds__(256) void plain_kernel(const PlainArgs P)
                     }
                 }
             }
+            // promote the word's FP32 partial sums
+#pragma unroll
+            for (int i = 0; i < PA; i++)
+#pragma unroll
+                for (int j = 0; j < PB; j++) {
+                    qd[i][j] += (double)qf[i][j];
+                    qf[i][j] = 0.0f;
+#pragma unroll
+                    for (int g = 0; g < NP; g++)
+#pragma unroll
+                        for (int h = 0; h < NP; h++) {
+                            sd[i][j][g][h] += (double)sf[i][j][g][h];
+                            sf[i][j][g][h] = 0.0f;
+                        }
+                }
         }
-        // promote the stage's FP32 partial sums
-#pragma unroll
-        for (int i = 0; i < PA; i++)
-#pragma unroll
-            for (int j = 0; j < PB; j++) {
-                qd[i][j] += (double)qf[i][j];
-                qf[i][j] = 0.0f;
-#pragma unroll
-                for (int g = 0; g < NP; g++)
-#pragma unroll
-                    for (int h = 0; h < NP; h++) {
-                        sd[i][j][g][h] += (double)sf[i][j][g][h];
-                        sf[i][j][g][h] = 0.0f;
-                    }
-            }
     }
 
 #pragma unroll
@@ -176,11 +179,11 @@ __global__ __launch_bounds__(256) void plain_kernel(const PlainArgs P)
 #pragma unroll
         for (int j = 0; j < PB; j++) {
             const int b = b0 + tx + 16 * j;
+            const Marg mb = P.marg[b < nsnp ? b : nsnp - 1];
             int xc[3][3];
             double xs[3][3];
             double sq;
             if (FAST) {
-                const Marg mb = P.marg[b < nsnp ? b : nsnp - 1];
 #pragma unroll
                 for (int g = 0; g < 2; g++) {
 #pragma unroll
@@ -188,9 +191,9 @@ __global__ __launch_bounds__(256) void plain_kernel(const PlainArgs P)
                         xc[g][h] = cnt[i][j][g][h];
                         xs[g][h] = sd[i][j][g][h];
                     }
-                    xc[g][2] = ma.n[g];
+                    xc[g][2] = ma.pn[g];
                     xs[g][2] = ma.ys[g];
-                    xc[2][g] = mb.n[g];
+                    xc[2][g] = mb.pn[g];
                     xs[2][g] = mb.ys[g];
                 }
                 xc[2][2] = P.nV;
@@ -211,7 +214,7 @@ __global__ __launch_bounds__(256) void plain_kernel(const PlainArgs P)
             cells_from_x(xc, n);
             cells_from_x(xs, s);
             const bool valid = a < a1 && b < b1 && a > b;
-            finish_pair<DUMP>(P.hs, P.ds, valid, a, b, n, s, sq);
+            finish_pair<DUMP>(P.hs, P.ds, valid, a, b, n, s, sq, ma.cls, mb.cls);
         }
     }
 }

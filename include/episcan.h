@@ -59,13 +59,15 @@ int epi_device_name(const epi_ctx *ctx, char *buf, size_t n);
  * words, npack1 = ceil(nid/16). Uploaded once and transcoded on the device
  * into class bitplanes (plain form) and class byte masks (MFMA form).
  * Replaces the per-chromosome clCreateBuffer/clEnqueueWriteBuffer of
- * src/EpiOpenCL.c:1530-1545. */
+ * src/EpiOpenCL.c:1530-1545. Resets the chromosome layout to one block over
+ * all SNPs; phenotypes of a different nid are dropped. */
 int epi_set_genotypes(epi_ctx *ctx, const int32_t *words, int32_t nid, int32_t nsnp);
 
 /* nphen phenotype vectors of nid FP32 values, row-major. A value of -9 (the
  * reference's sentinel, src/EpiOpenCL.c:135) or NaN is missing and removes
  * the individual from every pair for that phenotype. Replaces the phen
- * upload and mY/SSY of src/EpiOpenCL.c:1364-1376, :1507-1508. */
+ * upload and mY/SSY of src/EpiOpenCL.c:1364-1376, :1507-1508. nid must equal
+ * the genotypes' nid when genotypes are set. */
 int epi_set_phenotypes(epi_ctx *ctx, const float *phen, int32_t nphen, int32_t nid);
 
 /* Chromosome blocks (chromosome.chrstart / chrsize, src/readplink.h:34-39).
