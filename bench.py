@@ -87,11 +87,23 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-check", action="store_true", help="skip the oracle check")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run an elementwise copy of MIB MiB (FP32 y = x + 1) for HBM counter calibration")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        x = torch.zeros(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.add(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 3}))
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: the scan runs on one MI355X per process and this build has no "
-              "multi-GPU path (pairs shard as independent tile ranges; see DESIGN.md). Use --gpus 1.",
+              "multi-GPU path; pairs would shard as independent tile ranges over several processes). Use --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 
