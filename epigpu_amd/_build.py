@@ -62,6 +62,7 @@ def build_epihost(verbose=False):
 
 def build_oracle(verbose=False):
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], verbose)
+    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "dropin"], verbose)
 
 
 def build_all(verbose=False):

@@ -190,6 +190,24 @@ __device__ __forceinline__ void append_hit(const HitSink &hs, bool hit, const De
     }
 }
 
+// FP64 decision for a pair that passed the screen.
+__device__ __forceinline__ bool confirm64(const HitSink &hs, int a, int b, const int *n, const double *s, double sq,
+                                       DevHit &h)
+{
+    double F, F2;
+    int d1, d2, nfac;
+    stats64(n, s, sq, F, F2, d1, d2, nfac);
+    h.snp1 = a;
+    h.snp2 = b;
+    h.df1 = d1;
+    h.df2 = d2;
+    h.F = F;
+    h.F2 = hs.test == 'f' ? 0.0 : F2;
+    h.phen = hs.phen;
+    h.pad = 0;
+    return hit_rule(hs.test, F, F2, nfac, d1, d2, hs.t1, hs.t2);
+}
+
 // Finish one pair: counts n[9] (exact), sums s[9] of type S (exact integers
 // for the MFMA form, FP64 for the plain form), SQ in the same units squared.
 // Cells are indexed 3*pa + pb by plane; clsa/clsb map plane -> genotype class
@@ -226,20 +244,14 @@ __device__ __forceinline__ void finish_pair(const HitSink &hs, const DumpSink &d
     bool hit = false;
     DevHit h;
     if (cand) {
-        double F, F2;
-        int d1, d2, nfac;
-        stats64(n, s, sq, F, F2, d1, d2, nfac);
-        hit = hit_rule(hs.test, F, F2, nfac, d1, d2, hs.t1, hs.t2);
-        h.snp1 = a;
-        h.snp2 = b;
-        h.df1 = d1;
-        h.df2 = d2;
-        h.F = F;
-        h.F2 = hs.test == 'f' ? 0.0 : F2;
-        h.phen = hs.phen;
-        h.pad = 0;
+        double sd[9];
+#pragma unroll
+        for (int c = 0; c < 9; c++) sd[c] = (double)s[c];
+        hit = confirm64(hs, a, b, n, sd, sq, h);
     }
     append_hit(hs, hit, h);
+    // keep the scheduler from interleaving the temporaries of several pairs
+    __builtin_amdgcn_sched_barrier(0);
 }
 
 }  // namespace epi

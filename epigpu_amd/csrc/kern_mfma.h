@@ -159,17 +159,33 @@ __global__ __launch_bounds__(256, 1) void mfma_fast_kernel(const MfmaArgs A)
         }
     }
 
+    // Per-SNP class marginals of the tile, staged in LDS (the operand buffers
+    // are free once every wave has left the K loop).
+    __syncthreads();
+    int *mg_n = reinterpret_cast<int *>(lds);                    // [128][2]
+    long long *mg_q = reinterpret_cast<long long *>(lds + 1024); // [128][2]
+    if (tid < TA + TB) {
+        int sn = tid < TA ? a0 + tid : b0 + tid - TA;
+        sn = sn < A.nsnp ? sn : A.nsnp - 1;
+        const Marg &mm = A.marg[sn];
+        mg_n[2 * tid] = mm.n[0];
+        mg_n[2 * tid + 1] = mm.n[1];
+        mg_q[2 * tid] = mm.q[0];
+        mg_q[2 * tid + 1] = mm.q[1];
+    }
+    __syncthreads();
+
     // Epilogue: lane holds pairs (a = a0 + 16w + 4i + 2hf + p, b = b0 + 32bg + m).
 #pragma unroll
     for (int i = 0; i < 4; i++) {
 #pragma unroll
         for (int p = 0; p < 2; p++) {
-            const int a = a0 + 16 * wid + 4 * i + 2 * hf + p;
-            const Marg ma = A.marg[a < A.nsnp ? a : A.nsnp - 1];
+            const int al = 16 * wid + 4 * i + 2 * hf + p;
+            const int a = a0 + al;
 #pragma unroll
             for (int bg = 0; bg < 2; bg++) {
+                const int bl = TA + 32 * bg + m;
                 const int b = b0 + 32 * bg + m;
-                const Marg mb = A.marg[b < A.nsnp ? b : A.nsnp - 1];
                 int xc[3][3];
                 long long xs[3][3];
 #pragma unroll
@@ -181,10 +197,10 @@ __global__ __launch_bounds__(256, 1) void mfma_fast_kernel(const MfmaArgs A)
                         xc[g][h] = -v[e];
                         xs[g][h] = -((long long)v[e + 1] + 256ll * v[e + 2] + 65536ll * v[e + 3]);
                     }
-                    xc[g][2] = ma.n[g];
-                    xs[g][2] = ma.q[g];
-                    xc[2][g] = mb.n[g];
-                    xs[2][g] = mb.q[g];
+                    xc[g][2] = mg_n[2 * al + g];
+                    xs[g][2] = mg_q[2 * al + g];
+                    xc[2][g] = mg_n[2 * bl + g];
+                    xs[2][g] = mg_q[2 * bl + g];
                 }
                 xc[2][2] = A.nV;
                 xs[2][2] = A.qV;
