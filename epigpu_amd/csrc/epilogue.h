@@ -25,7 +25,7 @@ namespace epi {
 struct DevHit {
     int32_t snp1, snp2, df1, df2;
     double F, F2;
-    int32_t phen, pad;
+    int32_t tag, pad;  // tag: phenotype (single scan) or scan id (batch)
 };
 
 struct HitSink {
@@ -35,7 +35,7 @@ struct HitSink {
     int32_t test;
     float t1, t2;    // exact thresholds
     float t1s, t2s;  // screening thresholds (lowered)
-    int32_t phen;
+    int32_t tag;     // written into every hit
 };
 
 struct DumpSink {
@@ -203,50 +203,47 @@ __device__ __forceinline__ bool confirm64(const HitSink &hs, int a, int b, const
     h.df2 = d2;
     h.F = F;
     h.F2 = hs.test == 'f' ? 0.0 : F2;
-    h.phen = hs.phen;
+    h.tag = hs.tag;
     h.pad = 0;
     return hit_rule(hs.test, F, F2, nfac, d1, d2, hs.t1, hs.t2);
 }
 
-// Finish one pair: counts n[9] (exact), sums s[9] of type S (exact integers
-// for the MFMA form, FP64 for the plain form), SQ in the same units squared.
-// Cells are indexed 3*pa + pb by plane; clsa/clsb map plane -> genotype class
-// (2 bits per plane). The statistics do not depend on the class labels; only
-// the dump writes cells back in genotype order. valid = the pair is inside the
-// tile and a > b. All lanes must call.
-template <bool DUMP, class S>
+// Finish one pair. n[9] are the exact cell counts; sf[9] and sqf are FP32
+// cell sums and sum of squares, good enough to screen; exact(sd, sq) fills
+// the exact sums (FP64, same units) and is called only for pairs that pass the
+// screen, or for every pair when dumping. Cells are indexed 3*pa + pb by
+// plane; clsa/clsb map plane -> genotype class (2 bits per plane): the
+// statistics do not depend on the labels, only the dump writes cells back in
+// genotype order. valid = the pair is inside the tile and a > b. All lanes of
+// the wave must call.
+template <bool DUMP, class Exact>
 __device__ __forceinline__ void finish_pair(const HitSink &hs, const DumpSink &ds, bool valid, int a, int b,
-                                            const int n[9], const S s[9], double sq, int clsa, int clsb)
+                                            const int n[9], const float sf[9], float sqf, Exact exact, int clsa,
+                                            int clsb)
 {
     if (DUMP) {
         if (valid && a >= ds.a0 && a < ds.a1 && b >= ds.b0 && b < ds.b1) {
-            double F, F2;
+            double sd[9], sq, F, F2;
             int d1, d2, nfac;
-            stats64(n, s, sq, F, F2, d1, d2, nfac);
+            exact(sd, sq);
+            stats64(n, sd, sq, F, F2, d1, d2, nfac);
             long long p = (long long)(a - ds.a0) * (ds.b1 - ds.b0) + (b - ds.b0);
             for (int c = 0; c < 9; c++) {
                 int o = 3 * ((clsa >> (2 * (c / 3))) & 3) + ((clsb >> (2 * (c % 3))) & 3);
                 if (ds.counts) ds.counts[9 * p + o] = n[c];
-                if (ds.sums) ds.sums[9 * p + o] = (double)s[c] * ds.unit + (double)n[c] * ds.centre;
+                if (ds.sums) ds.sums[9 * p + o] = sd[c] * ds.unit + (double)n[c] * ds.centre;
             }
             if (ds.F) ds.F[p] = F;
             if (ds.F2) ds.F2[p] = F2;
         }
         return;
     }
-    bool cand = false;
-    if (valid) {
-        float sf[9];
-#pragma unroll
-        for (int c = 0; c < 9; c++) sf[c] = (float)s[c];
-        cand = screen32(n, sf, (float)sq, hs.test, hs.t1s, hs.t2s);
-    }
+    const bool cand = valid && screen32(n, sf, sqf, hs.test, hs.t1s, hs.t2s);
     bool hit = false;
     DevHit h;
     if (cand) {
-        double sd[9];
-#pragma unroll
-        for (int c = 0; c < 9; c++) sd[c] = (double)s[c];
+        double sd[9], sq;
+        exact(sd, sq);
         hit = confirm64(hs, a, b, n, sd, sq, h);
     }
     append_hit(hs, hit, h);

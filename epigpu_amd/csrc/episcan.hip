@@ -1,9 +1,10 @@
 // libepiscan.so - C ABI over the MI355X (gfx950) pair-scan kernels.
 //
-// Device state of one context:
+// Device state of one dataset (a context holds one; a batch holds many):
 //   words   [nsnp][npack1] int32   genotypes as uploaded (layout A1)
 //   planes  [nsnp][3][Kp]  uint8   class masks 0xFF/0x00 for classes 0, 1, NM
 //   bits    [3][Kw][nsnp]  uint32  class bitplanes (plain form)
+//   cls     [nsnp]         int32   plain-form plane -> class map
 //   per phenotype r:
 //     wts   [7][Kp] int8   V, 3 limbs of q, 3 limbs of round(q^2/2^23)
 //     qv    [Kp]    int32  fixed-point phenotype q (0 where missing)
@@ -22,16 +23,18 @@
 #include <vector>
 
 #include "../../include/episcan.h"
+#include "kern_batch.h"
 #include "kern_mfma.h"
 #include "kern_plain.h"
+#include "transcode.h"
 
 using namespace epi;
 
 namespace {
 
 constexpr int KPAD = 256;
-constexpr long long QMAX = 8355711;  // largest |q| with three balanced int8 limbs
-constexpr double ZSHIFT = 8388608.0; // 2^23
+constexpr long long QMAX = 8355711;   // largest |q| with three balanced int8 limbs
+constexpr double ZSHIFT = 8388608.0;  // 2^23
 
 struct PhenDev {
     int8_t *wts = nullptr;
@@ -39,45 +42,49 @@ struct PhenDev {
     float *yc = nullptr;
     uint32_t *vbits = nullptr;
     Marg *marg = nullptr;
-    int fbits = 0;        // q = round((y - centre) * 2^fbits)
+    int fbits = 0;  // q = round((y - centre) * 2^fbits)
     double centre = 0.0;
     int nV = 0;
     long long qV = 0;
-    double sqV = 0.0;     // sum q^2
-    double ysV = 0.0;     // sum yc
-    double y2V = 0.0;     // sum yc^2
+    double sqV = 0.0;  // sum q^2
+    double ysV = 0.0;  // sum yc
+    double y2V = 0.0;  // sum yc^2
+};
+
+struct Dataset {
+    int nid = 0, nsnp = 0, npack = 0, remain = 0, npack1 = 0, Kp = 0, Kw = 0;
+    int32_t *words = nullptr;
+    uint8_t *planes = nullptr;
+    uint32_t *bits = nullptr;
+    int32_t *dmiss = nullptr;
+    int32_t *cls = nullptr;
+    std::vector<int> miss;   // per SNP: any missing call
+    std::vector<int> missc;  // prefix count of SNPs with missing calls
+    std::vector<PhenDev> ph;
+    bool marg_ok = false;
+    std::vector<int> chrstart, chrsize, snpchr;
+};
+
+struct TileLists {
+    std::vector<int4> fast, gen;
+    long long pairs = 0;
 };
 
 }  // namespace
 
 struct epi_ctx {
     int device = 0;
+    int ncu = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
     char name[256] = {0};
-    // genotypes
-    int nid = 0, nsnp = 0, npack = 0, remain = 0, npack1 = 0, Kp = 0, Kw = 0;
-    int32_t *words = nullptr;
-    uint8_t *planes = nullptr;
-    uint32_t *bits = nullptr;
-    int32_t *dmiss = nullptr;
-    int32_t *cls = nullptr;  // per SNP plane -> class map of the plain form
-    std::vector<int> miss;   // per SNP: any missing call
-    std::vector<int> missc;  // prefix count of SNPs with missing calls
-    // phenotypes
-    int nphen = 0;
-    std::vector<PhenDev> ph;
-    bool marg_ok = false;
-    // chromosomes
-    std::vector<int> chrstart, chrsize;
-    std::vector<int> snpchr;
+    Dataset d;
     // hits
     DevHit *hits = nullptr;
     unsigned long long *nhits = nullptr;
     long long cap = 1ll << 22;
     long long last_nhits = 0;
-    int last_test = 'f';
     // tiles
     int4 *tiles = nullptr;
     size_t tiles_cap = 0;
@@ -85,6 +92,40 @@ struct epi_ctx {
     double last_ms = 0.0;
     long long last_pairs = 0;
     int last_launches = 0;
+    epi_schedule sched[2];  // per form
+};
+
+struct epi_batch {
+    epi_ctx *ctx = nullptr;
+    std::vector<Dataset *> data;
+    std::vector<std::pair<int, int>> scans;  // (dataset, phenotype)
+    MfmaScan *d_mscan = nullptr;
+    PlainScan *d_pscan = nullptr;
+    int4 *d_work = nullptr;
+    int *d_qpos = nullptr, *d_qbegin = nullptr, *d_qend = nullptr;
+    size_t work_cap = 0, scan_cap = 0;
+    DevHit *hits = nullptr;
+    unsigned long long *nhits = nullptr;
+    long long cap = 1ll << 22;
+    // work list of the last build
+    epi_schedule sched[2];  // per form
+    epi_schedule work_sched = {};
+    bool dirty = true;
+    int work_form = -1;
+    long long nwork = 0, pairs = 0;
+    // captured graph of the last scan
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    epi_scan_params gkey = {};
+    bool gvalid = false;
+    int captures = 0;
+    int grid = 0;
+    // results
+    long long last_nhits = 0;
+    double last_ms = 0.0;
+    bool fetched = false;
+    std::vector<DevHit> hh;
+    std::vector<std::vector<int>> by_scan;
 };
 
 namespace {
@@ -102,12 +143,12 @@ int fail(epi_ctx *c, int code, const char *fmt, ...)
     return code;
 }
 
-#define HIPCHK(c, call)                                                                        \
-    do {                                                                                       \
-        hipError_t e_ = (call);                                                                \
-        if (e_ != hipSuccess)                                                                  \
+#define HIPCHK(c, call)                                                                          \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
             return fail((c), EPI_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
-                        __FILE__, __LINE__);                                                   \
+                        __FILE__, __LINE__);                                                     \
     } while (0)
 
 template <class T>
@@ -125,8 +166,7 @@ int dalloc(epi_ctx *c, T *&p, size_t count)
     hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
     if (e != hipSuccess) {
         p = nullptr;
-        return fail(c, EPI_ERR_NOMEM, "hipMalloc of %zu bytes failed: %s", count * sizeof(T),
-                    hipGetErrorString(e));
+        return fail(c, EPI_ERR_NOMEM, "hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
     }
     return EPI_OK;
 }
@@ -140,184 +180,298 @@ void free_phen(PhenDev &p)
     dfree(p.marg);
 }
 
-// ------------------------------------------------------------ kernels ----
-
-__device__ __forceinline__ int decode(const uint32_t *w, int npack, int remain, int nid, int k)
+void free_dataset(Dataset &d)
 {
-    if (k >= nid) return -1;
-    int j = k >> 4, t = k & 15;
-    int sh = j < npack ? 30 - 2 * t : 2 * (remain - 1 - t);
-    return (int)((w[j] >> sh) & 3u);
+    dfree(d.words);
+    dfree(d.planes);
+    dfree(d.bits);
+    dfree(d.dmiss);
+    dfree(d.cls);
+    for (auto &p : d.ph) free_phen(p);
+    d.ph.clear();
 }
 
-// One thread per (snp, 16-individual group): class masks for the MFMA form.
-__global__ void transcode_planes(const uint32_t *words, int nid, int npack, int remain, int npack1,
-                                 int nsnp, int Kp, uint8_t *planes, int32_t *miss)
-{
-    const int groups = Kp / 16;
-    long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= (long long)groups * nsnp) return;
-    int s = (int)(gid / groups), grp = (int)(gid % groups);
-    const uint32_t *w = words + (size_t)s * npack1;
-    uint32_t m[3][4] = {{0}};
-    int anymiss = 0;
-    for (int t = 0; t < 16; t++) {
-        int c = decode(w, npack, remain, nid, 16 * grp + t);
-        uint32_t byte = 0xFFu << (8 * (t & 3));
-        if (c == 0) m[0][t >> 2] |= byte;
-        if (c == 1) m[1][t >> 2] |= byte;
-        if (c >= 0 && c != 3) m[2][t >> 2] |= byte;
-        anymiss |= c == 3;
-    }
-    for (int p = 0; p < 3; p++)
-        *reinterpret_cast<uint4 *>(planes + ((size_t)s * 3 + p) * Kp + 16 * grp) =
-            make_uint4(m[p][0], m[p][1], m[p][2], m[p][3]);
-    if (anymiss) atomicOr(&miss[s], 1);
-}
+// ------------------------------------------------------ dataset preparation --
 
-// One thread per SNP: order the genotype classes by frequency. The plain form
-// stores the two rarest classes as bitplanes 0 and 1 and derives the most
-// common one; cls packs plane -> class (2 bits each).
-__global__ void class_order(const uint32_t *words, int nid, int npack, int remain, int npack1, int nsnp,
-                            int32_t *cls)
+int ds_set_genotypes(epi_ctx *c, Dataset &d, const int32_t *words, int nid, int nsnp)
 {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nsnp) return;
-    const uint32_t *w = words + (size_t)s * npack1;
-    int n[4] = {0, 0, 0, 0};
-    for (int k = 0; k < nid; k++) n[decode(w, npack, remain, nid, k)]++;
-    int big = 0;
-    if (n[1] > n[big]) big = 1;
-    if (n[2] > n[big]) big = 2;
-    int p0 = big == 0 ? 1 : 0;
-    int p1 = big == 2 ? 1 : 2;
-    cls[s] = p0 | (p1 << 2) | (big << 4);
-}
-
-// One thread per (word of 32 individuals, snp): class bitplanes, plain form.
-__global__ void transcode_bits(const uint32_t *words, int nid, int npack, int remain, int npack1, int nsnp,
-                               int Kw, const int32_t *cls, uint32_t *bits)
-{
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    int wd = blockIdx.y;
-    if (s >= nsnp) return;
-    const uint32_t *w = words + (size_t)s * npack1;
-    const int c0 = cls[s] & 3, c1 = (cls[s] >> 2) & 3;
-    uint32_t b0 = 0, b1 = 0, bn = 0;
-    for (int t = 0; t < 32; t++) {
-        int c = decode(w, npack, remain, nid, 32 * wd + t);
-        b0 |= (uint32_t)(c == c0) << t;
-        b1 |= (uint32_t)(c == c1) << t;
-        bn |= (uint32_t)(c >= 0 && c != 3) << t;
+    if (!words || nid < 2 || nsnp < 2) return fail(c, EPI_ERR_ARG, "need nid >= 2, nsnp >= 2 and words");
+    if (!d.ph.empty() && nid != d.nid) {  // a new cohort: phenotypes of the old one no longer apply
+        for (auto &p : d.ph) free_phen(p);
+        d.ph.clear();
     }
-    bits[((size_t)0 * Kw + wd) * nsnp + s] = b0;
-    bits[((size_t)1 * Kw + wd) * nsnp + s] = b1;
-    bits[((size_t)2 * Kw + wd) * nsnp + s] = bn;
-}
-
-// One workgroup per SNP: per genotype class (0, 1, 2) counts and phenotype
-// sums over present phenotypes, for one phenotype; stored per class for the
-// MFMA form and per bitplane (cls order) for the plain form.
-__global__ __launch_bounds__(256) void marginals_kernel(const uint8_t *planes, int Kp, int nsnp,
-                                                        const int8_t *wts, const int32_t *qv,
-                                                        const float *yc, const int32_t *cls, Marg *marg)
-{
-    __shared__ long long sq[3][256];
-    __shared__ double sy[3][256];
-    __shared__ int sn[3][256];
-    const int s = blockIdx.x, tid = threadIdx.x;
-    int n[3] = {0, 0, 0};
-    long long q[3] = {0, 0, 0};
-    double y[3] = {0.0, 0.0, 0.0};
-    const uint8_t *pl = planes + (size_t)s * 3 * Kp;
-    for (int k = tid; k < Kp; k += 256) {
-        if (!wts[k]) continue;  // row 0 = V
-        if (!pl[(size_t)2 * Kp + k]) continue;  // missing call
-        int c = pl[k] ? 0 : (pl[(size_t)Kp + k] ? 1 : 2);
-        n[c] += 1;
-        q[c] += qv[k];
-        y[c] += (double)yc[k];
-    }
-    for (int p = 0; p < 3; p++) {
-        sn[p][tid] = n[p];
-        sq[p][tid] = q[p];
-        sy[p][tid] = y[p];
-    }
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-        if (tid < st)
-            for (int p = 0; p < 3; p++) {
-                sn[p][tid] += sn[p][tid + st];
-                sq[p][tid] += sq[p][tid + st];
-                sy[p][tid] += sy[p][tid + st];
-            }
-        __syncthreads();
-    }
-    if (tid == 0) {
-        Marg m;
-        const int cl = cls[s];
-        const int ntot = sn[0][0] + sn[1][0] + sn[2][0];
-        m.n[0] = sn[0][0];
-        m.n[1] = sn[1][0];
-        m.n[2] = ntot;
-        m.q[0] = sq[0][0];
-        m.q[1] = sq[1][0];
-        m.q[2] = sq[0][0] + sq[1][0] + sq[2][0];
-        for (int p = 0; p < 2; p++) {
-            int c = (cl >> (2 * p)) & 3;
-            m.pn[p] = sn[c][0];
-            m.ys[p] = sy[c][0];
-        }
-        m.pn[2] = ntot;
-        m.ys[2] = sy[0][0] + sy[1][0] + sy[2][0];
-        m.cls = cl;
-        m.pad = 0;
-        marg[s] = m;
-    }
-}
-
-// ---------------------------------------------------------- host side ----
-
-int ensure_marginals(epi_ctx *c)
-{
-    if (c->marg_ok) return EPI_OK;
-    if (!c->planes || c->nphen == 0) return fail(c, EPI_ERR_STATE, "genotypes and phenotypes must be set");
-    for (int r = 0; r < c->nphen; r++) {
-        PhenDev &p = c->ph[r];
-        int rc = dalloc(c, p.marg, (size_t)c->nsnp);
-        if (rc) return rc;
-        hipLaunchKernelGGL(marginals_kernel, dim3(c->nsnp), dim3(256), 0, c->stream, c->planes, c->Kp,
-                           c->nsnp, p.wts, p.qv, p.yc, c->cls, p.marg);
-        HIPCHK(c, hipGetLastError());
-    }
+    d.nid = nid;
+    d.nsnp = nsnp;
+    d.npack = nid / 16;
+    d.remain = nid % 16;
+    d.npack1 = d.npack + (d.remain > 0);
+    d.Kp = (nid + KPAD - 1) / KPAD * KPAD;
+    d.Kw = d.Kp / 32;
+    size_t nwords = (size_t)nsnp * d.npack1;
+    int rc;
+    if ((rc = dalloc(c, d.words, nwords)) || (rc = dalloc(c, d.planes, (size_t)nsnp * 3 * d.Kp)) ||
+        (rc = dalloc(c, d.bits, (size_t)3 * d.Kw * nsnp)) || (rc = dalloc(c, d.dmiss, (size_t)nsnp)) ||
+        (rc = dalloc(c, d.cls, (size_t)nsnp)))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(d.words, words, nwords * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(d.dmiss, 0, (size_t)nsnp * sizeof(int32_t), c->stream));
+    long long total = (long long)(d.Kp / 16) * nsnp;
+    hipLaunchKernelGGL(transcode_planes, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, c->stream,
+                       (const uint32_t *)d.words, nid, d.npack, d.remain, d.npack1, nsnp, d.Kp, d.planes, d.dmiss);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(class_order, dim3((nsnp + 255) / 256), dim3(256), 0, c->stream, (const uint32_t *)d.words, nid,
+                       d.npack, d.remain, d.npack1, nsnp, d.cls);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(transcode_bits, dim3((nsnp + 255) / 256, d.Kw), dim3(256), 0, c->stream,
+                       (const uint32_t *)d.words, nid, d.npack, d.remain, d.npack1, nsnp, d.Kw, d.cls, d.bits);
+    HIPCHK(c, hipGetLastError());
+    d.miss.assign(nsnp, 0);
+    HIPCHK(c, hipMemcpyAsync(d.miss.data(), d.dmiss, (size_t)nsnp * sizeof(int32_t), hipMemcpyDeviceToHost,
+                             c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    c->marg_ok = true;
+    d.missc.assign(nsnp + 1, 0);
+    for (int s = 0; s < nsnp; s++) d.missc[s + 1] = d.missc[s] + (d.miss[s] != 0);
+    d.marg_ok = false;
+    // chromosome layout resets to one block over all SNPs
+    d.chrstart.assign(1, 0);
+    d.chrsize.assign(1, nsnp);
+    d.snpchr.assign(nsnp, 0);
     return EPI_OK;
 }
 
-struct TileLists {
-    std::vector<int4> fast, gen;
-    long long pairs = 0;
-};
-
-// Tiles of shape TA x TB over the block range, split into FAST tiles (no
-// missing call in either SNP range) and GENERAL tiles of shape (TA/split) x TB.
-// Within a block the tiles are emitted in super-tiles of 8 x 4 tiles so that
-// neighbouring workgroups share SNP data in L2.
-void build_tiles(const epi_ctx *c, int blk_begin, int blk_end, int TA, int TB, int split, TileLists &out)
+int ds_set_phenotypes(epi_ctx *c, Dataset &d, const float *phen, int nphen, int nid)
 {
-    const int nchr = (int)c->chrstart.size();
-    auto anymiss = [&](int s0, int s1) { return c->missc[s1] - c->missc[s0] > 0; };
-    const int SA = 8, SB = 4;
+    if (!phen || nphen < 1 || nid < 2) return fail(c, EPI_ERR_ARG, "need phenotypes");
+    if (d.nid && nid != d.nid) return fail(c, EPI_ERR_ARG, "nid %d differs from genotypes' nid %d", nid, d.nid);
+    const int Kp = (nid + KPAD - 1) / KPAD * KPAD, Kw = Kp / 32;
+    for (auto &p : d.ph) free_phen(p);
+    d.ph.assign(nphen, PhenDev());
+    if (!d.nid) {
+        d.nid = nid;
+        d.Kp = Kp;
+        d.Kw = Kw;
+    }
+    std::vector<int8_t> w((size_t)NW * Kp);
+    std::vector<int32_t> qv(Kp);
+    std::vector<float> yc(Kp);
+    std::vector<uint32_t> vb(Kw);
+    for (int r = 0; r < nphen; r++) {
+        const float *y = phen + (size_t)r * nid;
+        PhenDev &p = d.ph[r];
+        double sum = 0.0;
+        int nv = 0;
+        for (int k = 0; k < nid; k++) {
+            float v = y[k];
+            if (v != v || v == -9.0f) continue;
+            if (std::isinf(v)) return fail(c, EPI_ERR_RANGE, "phenotype %d has an infinite value", r);
+            sum += v;
+            nv++;
+        }
+        if (nv < 2) return fail(c, EPI_ERR_RANGE, "phenotype %d has fewer than 2 present values", r);
+        double centre = sum / nv, maxabs = 0.0;
+        for (int k = 0; k < nid; k++) {
+            float v = y[k];
+            if (v != v || v == -9.0f) continue;
+            maxabs = std::max(maxabs, std::fabs((double)v - centre));
+        }
+        int f = 0;
+        if (maxabs > 0.0) f = (int)std::floor(std::log2((double)QMAX / maxabs));
+        while (maxabs > 0.0 && std::ldexp(maxabs, f) > (double)QMAX) f--;
+        std::fill(w.begin(), w.end(), 0);
+        std::fill(qv.begin(), qv.end(), 0);
+        std::fill(yc.begin(), yc.end(), 0.0f);
+        std::fill(vb.begin(), vb.end(), 0u);
+        long long qsum = 0;
+        double sq = 0.0, ys = 0.0, y2 = 0.0;
+        for (int k = 0; k < nid; k++) {
+            float v = y[k];
+            if (v != v || v == -9.0f) continue;
+            double dv = (double)v - centre;
+            long long q = std::llround(std::ldexp(dv, f));
+            if (q > QMAX || q < -QMAX) return fail(c, EPI_ERR_RANGE, "fixed-point phenotype out of range");
+            long long l0 = ((q % 256) + 256 + 128) % 256 - 128;
+            long long q1 = (q - l0) / 256;
+            long long l1 = ((q1 % 256) + 256 + 128) % 256 - 128;
+            long long l2 = (q1 - l1) / 256;
+            long long z = std::llround((double)q * (double)q / ZSHIFT);
+            long long z0 = ((z % 256) + 256 + 128) % 256 - 128;
+            long long z1q = (z - z0) / 256;
+            long long z1 = ((z1q % 256) + 256 + 128) % 256 - 128;
+            long long z2 = (z1q - z1) / 256;
+            if (l2 < -128 || l2 > 127 || z2 < -128 || z2 > 127) return fail(c, EPI_ERR_RANGE, "limb overflow");
+            w[0 * (size_t)Kp + k] = 1;
+            w[1 * (size_t)Kp + k] = (int8_t)l0;
+            w[2 * (size_t)Kp + k] = (int8_t)l1;
+            w[3 * (size_t)Kp + k] = (int8_t)l2;
+            w[4 * (size_t)Kp + k] = (int8_t)z0;
+            w[5 * (size_t)Kp + k] = (int8_t)z1;
+            w[6 * (size_t)Kp + k] = (int8_t)z2;
+            qv[k] = (int32_t)q;
+            qsum += q;
+            sq += (double)q * (double)q;
+            float ycf = (float)dv;
+            yc[k] = ycf;
+            ys += (double)ycf;
+            y2 += (double)ycf * (double)ycf;
+            vb[k >> 5] |= 1u << (k & 31);
+        }
+        p.fbits = f;
+        p.centre = centre;
+        p.nV = nv;
+        p.qV = qsum;
+        p.sqV = sq;
+        p.ysV = ys;
+        p.y2V = y2;
+        int rc;
+        if ((rc = dalloc(c, p.wts, w.size())) || (rc = dalloc(c, p.qv, (size_t)Kp)) ||
+            (rc = dalloc(c, p.yc, (size_t)Kp)) || (rc = dalloc(c, p.vbits, (size_t)Kw)))
+            return rc;
+        HIPCHK(c, hipMemcpyAsync(p.wts, w.data(), w.size(), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(p.qv, qv.data(), (size_t)Kp * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(p.yc, yc.data(), (size_t)Kp * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(p.vbits, vb.data(), (size_t)Kw * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    d.marg_ok = false;
+    return EPI_OK;
+}
+
+int ds_set_chromosomes(epi_ctx *c, Dataset &d, const int32_t *chrstart, const int32_t *chrsize, int nchr)
+{
+    if (!chrstart || !chrsize || nchr < 1) return fail(c, EPI_ERR_ARG, "need chromosomes");
+    if (!d.nsnp) return fail(c, EPI_ERR_STATE, "set genotypes before chromosomes");
+    std::vector<int> snpchr(d.nsnp, -1);
+    for (int i = 0; i < nchr; i++) {
+        if (chrsize[i] < 0 || chrstart[i] < 0 || (long long)chrstart[i] + chrsize[i] > d.nsnp)
+            return fail(c, EPI_ERR_ARG, "chromosome %d out of range", i);
+        for (int s = chrstart[i]; s < chrstart[i] + chrsize[i]; s++) {
+            if (snpchr[s] >= 0) return fail(c, EPI_ERR_ARG, "chromosomes %d and %d overlap", snpchr[s], i);
+            snpchr[s] = i;
+        }
+    }
+    d.chrstart.assign(chrstart, chrstart + nchr);
+    d.chrsize.assign(chrsize, chrsize + nchr);
+    d.snpchr.swap(snpchr);
+    return EPI_OK;
+}
+
+int ds_marginals(epi_ctx *c, Dataset &d)
+{
+    if (d.marg_ok) return EPI_OK;
+    if (!d.planes || d.ph.empty()) return fail(c, EPI_ERR_STATE, "genotypes and phenotypes must be set");
+    for (auto &p : d.ph) {
+        int rc = dalloc(c, p.marg, (size_t)d.nsnp);
+        if (rc) return rc;
+        hipLaunchKernelGGL(marginals_kernel, dim3(d.nsnp), dim3(256), 0, c->stream, d.planes, d.Kp, d.nsnp, p.wts,
+                           p.qv, p.yc, d.cls, p.marg);
+        HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    d.marg_ok = true;
+    return EPI_OK;
+}
+
+MfmaScan mfma_scan(const Dataset &d, int r, int tag)
+{
+    const PhenDev &p = d.ph[r];
+    MfmaScan S;
+    S.planes = d.planes;
+    S.wts = p.wts;
+    S.marg = p.marg;
+    S.nsnp = d.nsnp;
+    S.Kp = d.Kp;
+    S.nV = p.nV;
+    S.tag = tag;
+    S.qV = p.qV;
+    S.qVd = (double)p.qV;
+    S.sqV = p.sqV;
+    return S;
+}
+
+PlainScan plain_scan(const Dataset &d, int r, int tag)
+{
+    const PhenDev &p = d.ph[r];
+    PlainScan S;
+    S.bits = d.bits;
+    S.yc = p.yc;
+    S.vbits = p.vbits;
+    S.marg = p.marg;
+    S.nsnp = d.nsnp;
+    S.Kw = d.Kw;
+    S.nV = p.nV;
+    S.tag = tag;
+    S.ysV = p.ysV;
+    S.y2V = p.y2V;
+    return S;
+}
+
+// Schedules. FAST tiles (no missing call in either SNP range) are
+// fast_rows x 64. GENERAL tiles are 16 x 64, except 16 x 32 in the MFMA form
+// with 32-row FAST tiles (so that both fit two workgroups per CU).
+constexpr int TILE_B = 64, GEN_ROWS = 16;
+
+int gen_cols(int form, const epi_schedule &s) { return form == EPI_FORM_MFMA && s.fast_rows == 32 ? 32 : 64; }
+
+epi_schedule default_schedule(int form)
+{
+    epi_schedule s;
+    memset(&s, 0, sizeof(s));
+    s.fast_rows = form == EPI_FORM_MFMA ? 64 : 32;
+    s.super_a = 8;
+    s.super_b = 4;
+    s.queues = 8;
+    s.order = 0;
+    return s;
+}
+
+bool valid_schedule(int form, const epi_schedule &s)
+{
+    bool rows = form == EPI_FORM_MFMA ? (s.fast_rows == 64 || s.fast_rows == 32)
+                                      : (s.fast_rows == 32 || s.fast_rows == 16);
+    return rows && s.super_a >= 1 && s.super_a <= 64 && s.super_b >= 1 && s.super_b <= 64 &&
+           (s.queues == 1 || s.queues == 8) && s.order >= 0 && s.order <= 2;
+}
+
+bool same_schedule(const epi_schedule &x, const epi_schedule &y)
+{
+    return x.fast_rows == y.fast_rows && x.super_a == y.super_a && x.super_b == y.super_b && x.queues == y.queues &&
+           x.order == y.order;
+}
+
+void emit_tile(const Dataset &d, int a0, int a1, int b0, int b1, int gcols, TileLists &out)
+{
+    auto anymiss = [&](int s0, int s1) { return d.missc[s1] - d.missc[s0] > 0; };
+    if (a1 - 1 <= b0) return;  // no pair with a > b
+    if (!anymiss(a0, a1) && !anymiss(b0, b1)) {
+        out.fast.push_back(make_int4(a0, a1, b0, b1));
+        return;
+    }
+    for (int x = a0; x < a1; x += GEN_ROWS) {
+        int x1 = std::min(a1, x + GEN_ROWS);
+        for (int y = b0; y < b1; y += gcols) {
+            int y1 = std::min(b1, y + gcols);
+            if (x1 - 1 <= y) continue;
+            out.gen.push_back(make_int4(x, x1, y, y1));
+        }
+    }
+}
+
+// Tiles over the chr x chr blocks [blk_begin, blk_end). Within a block the
+// tiles are emitted in super-tiles of super_a x super_b tiles so that
+// neighbouring workgroups share SNP data in L2.
+void build_tiles(const Dataset &d, int blk_begin, int blk_end, int form, const epi_schedule &sc, TileLists &out)
+{
+    const int TA = sc.fast_rows, TB = TILE_B, gcols = gen_cols(form, sc);
+    const int nchr = (int)d.chrstart.size();
+    const int SA = sc.super_a, SB = sc.super_b;
     for (int i = 0; i < nchr; i++) {
         for (int j = 0; j <= i; j++) {
             int blk = i * (i + 1) / 2 + j;
             if (blk < blk_begin || blk >= blk_end) continue;
-            int A0 = c->chrstart[i], A1 = A0 + c->chrsize[i];
-            int B0 = c->chrstart[j], B1 = B0 + c->chrsize[j];
+            int A0 = d.chrstart[i], A1 = A0 + d.chrsize[i];
+            int B0 = d.chrstart[j], B1 = B0 + d.chrsize[j];
             if (A1 <= A0 || B1 <= B0) continue;
-            // pairs in this block (a > b)
-            for (long long a = A0; a < A1; a++) {
+            for (long long a = A0; a < A1; a++) {  // pairs in this block (a > b)
                 long long lo = B0, hi = std::min<long long>(B1, a);
                 if (hi > lo) out.pairs += hi - lo;
             }
@@ -328,17 +482,7 @@ void build_tiles(const epi_ctx *c, int blk_begin, int blk_end, int TA, int TB, i
                         for (int tb = sb; tb < std::min(ntb, sb + SB); tb++) {
                             int a0 = A0 + ta * TA, a1 = std::min(A1, a0 + TA);
                             int b0 = B0 + tb * TB, b1 = std::min(B1, b0 + TB);
-                            if (a1 - 1 <= b0) continue;  // no pair with a > b
-                            if (!anymiss(a0, a1) && !anymiss(b0, b1)) {
-                                out.fast.push_back(make_int4(a0, a1, b0, b1));
-                            } else {
-                                int sub = TA / split;
-                                for (int x = a0; x < a1; x += sub) {
-                                    int x1 = std::min(a1, x + sub);
-                                    if (x1 - 1 <= b0) continue;
-                                    out.gen.push_back(make_int4(x, x1, b0, b1));
-                                }
-                            }
+                            emit_tile(d, a0, a1, b0, b1, gcols, out);
                         }
         }
     }
@@ -361,83 +505,86 @@ int upload_tiles(epi_ctx *c, const TileLists &tl)
     return EPI_OK;
 }
 
-HitSink make_sink(epi_ctx *c, const epi_scan_params *p)
+HitSink make_sink(DevHit *hits, unsigned long long *nhits, long long cap, const epi_scan_params *p)
 {
     HitSink hs;
-    hs.hits = c->hits;
-    hs.nhits = c->nhits;
-    hs.cap = c->cap;
+    hs.hits = hits;
+    hs.nhits = nhits;
+    hs.cap = cap;
     hs.test = p->test;
     hs.t1 = p->t1;
     hs.t2 = p->t2;
     hs.t1s = p->t1 * 0.98f;
     hs.t2s = p->t2 * 0.98f;
-    hs.phen = p->phen;
+    hs.tag = p->phen;
     return hs;
 }
 
-int check_params(epi_ctx *c, const epi_scan_params *p)
+int check_common(epi_ctx *c, const epi_scan_params *p)
 {
     if (!p) return fail(c, EPI_ERR_ARG, "null params");
     if (p->test != 'f' && p->test != 'i') return fail(c, EPI_ERR_ARG, "test must be 'f' or 'i'");
     if (p->form != EPI_FORM_PLAIN && p->form != EPI_FORM_MFMA) return fail(c, EPI_ERR_ARG, "unknown form");
-    if (p->phen < 0 || p->phen >= c->nphen) return fail(c, EPI_ERR_ARG, "phenotype index out of range");
     if (!(p->t1 == p->t1) || !(p->t2 == p->t2)) return fail(c, EPI_ERR_ARG, "threshold is NaN");
-    if (!c->planes) return fail(c, EPI_ERR_STATE, "genotypes not set");
-    if (c->chrstart.empty()) return fail(c, EPI_ERR_STATE, "chromosomes not set");
     return EPI_OK;
 }
 
-// Launch the pair kernels over the uploaded tile lists (fast tiles first).
-template <bool DUMP>
-int launch_pairs(epi_ctx *c, const epi_scan_params *p, const TileLists &tl, const DumpSink &ds)
+int check_params(epi_ctx *c, const epi_scan_params *p)
 {
-    PhenDev &ph = c->ph[p->phen];
+    int rc = check_common(c, p);
+    if (rc) return rc;
+    if (!c->d.planes) return fail(c, EPI_ERR_STATE, "genotypes not set");
+    if (p->phen < 0 || p->phen >= (int)c->d.ph.size()) return fail(c, EPI_ERR_ARG, "phenotype index out of range");
+    if (c->d.chrstart.empty()) return fail(c, EPI_ERR_STATE, "chromosomes not set");
+    return EPI_OK;
+}
+
+// Launch the pair kernels of one scan over the uploaded tile lists (fast tiles first).
+template <bool DUMP>
+int launch_pairs(epi_ctx *c, const epi_scan_params *p, const epi_schedule &sc, const TileLists &tl,
+                 const DumpSink &ds)
+{
+    const Dataset &d = c->d;
     const int nf = (int)tl.fast.size(), ng = (int)tl.gen.size();
     c->last_launches = 0;
+    const HitSink hs = make_sink(c->hits, c->nhits, c->cap, p);
     if (p->form == EPI_FORM_MFMA) {
         MfmaArgs A;
-        A.planes = c->planes;
-        A.wts = ph.wts;
-        A.marg = ph.marg;
-        A.nsnp = c->nsnp;
-        A.Kp = c->Kp;
-        A.nV = ph.nV;
-        A.qV = ph.qV;
-        A.sqV = ph.sqV;
-        A.hs = make_sink(c, p);
+        A.S = mfma_scan(d, p->phen, p->phen);
+        A.hs = hs;
         A.ds = ds;
         if (nf) {
             A.tiles = c->tiles;
             A.ntiles = nf;
-            hipLaunchKernelGGL(mfma_fast_kernel<DUMP>, dim3(nf), dim3(256), 0, c->stream, A);
+            if (sc.fast_rows == 64)
+                hipLaunchKernelGGL((mfma_fast_kernel<64, DUMP>), dim3(nf), dim3(256), 0, c->stream, A);
+            else
+                hipLaunchKernelGGL((mfma_fast_kernel<32, DUMP>), dim3(nf), dim3(256), 0, c->stream, A);
             HIPCHK(c, hipGetLastError());
             c->last_launches++;
         }
         if (ng) {
             A.tiles = c->tiles + nf;
             A.ntiles = ng;
-            hipLaunchKernelGGL(mfma_general_kernel<DUMP>, dim3(ng), dim3(256), 0, c->stream, A);
+            if (sc.fast_rows == 64)
+                hipLaunchKernelGGL((mfma_general_kernel<2, DUMP>), dim3(ng), dim3(256), 0, c->stream, A);
+            else
+                hipLaunchKernelGGL((mfma_general_kernel<1, DUMP>), dim3(ng), dim3(256), 0, c->stream, A);
             HIPCHK(c, hipGetLastError());
             c->last_launches++;
         }
     } else {
         PlainArgs P;
-        P.bits = c->bits;
-        P.yc = ph.yc;
-        P.vbits = ph.vbits;
-        P.marg = ph.marg;
-        P.nsnp = c->nsnp;
-        P.Kw = c->Kw;
-        P.nV = ph.nV;
-        P.ysV = ph.ysV;
-        P.y2V = ph.y2V;
-        P.hs = make_sink(c, p);
+        P.S = plain_scan(d, p->phen, p->phen);
+        P.hs = hs;
         P.ds = ds;
         if (nf) {
             P.tiles = c->tiles;
             P.ntiles = nf;
-            hipLaunchKernelGGL((plain_kernel<2, 4, true, DUMP>), dim3(nf), dim3(256), 0, c->stream, P);
+            if (sc.fast_rows == 32)
+                hipLaunchKernelGGL((plain_kernel<2, 4, true, DUMP>), dim3(nf), dim3(256), 0, c->stream, P);
+            else
+                hipLaunchKernelGGL((plain_kernel<1, 4, true, DUMP>), dim3(nf), dim3(256), 0, c->stream, P);
             HIPCHK(c, hipGetLastError());
             c->last_launches++;
         }
@@ -452,17 +599,262 @@ int launch_pairs(epi_ctx *c, const epi_scan_params *p, const TileLists &tl, cons
     return EPI_OK;
 }
 
-void tile_shape(int form, int &TA, int &TB, int &split)
+// Hits of one scan sorted by (block, snp1, snp2) and written out.
+void emit_hits(const Dataset &d, const std::vector<DevHit> &h, const std::vector<int> &idx, epi_hit *hits, double *F64,
+               double *F2_64, int32_t *block, long long cap, long long *n)
 {
-    if (form == EPI_FORM_MFMA) {
-        TA = 64;
-        TB = 64;
-        split = 4;
-    } else {
-        TA = 32;
-        TB = 64;
-        split = 2;
+    auto blk_of = [&](const DevHit &x) {
+        int i = d.snpchr[x.snp1], j = d.snpchr[x.snp2];
+        return i * (i + 1) / 2 + j;
+    };
+    std::vector<int> key(idx);
+    std::sort(key.begin(), key.end(), [&](int x, int y) {
+        const DevHit &p = h[x], &q = h[y];
+        int bp = blk_of(p), bq = blk_of(q);
+        if (bp != bq) return bp < bq;
+        if (p.snp1 != q.snp1) return p.snp1 < q.snp1;
+        if (p.snp2 != q.snp2) return p.snp2 < q.snp2;
+        return p.tag < q.tag;
+    });
+    long long m = std::min<long long>((long long)key.size(), cap);
+    for (long long t = 0; t < m; t++) {
+        const DevHit &x = h[key[t]];
+        if (hits) {
+            hits[t].snp1 = x.snp1;
+            hits[t].snp2 = x.snp2;
+            hits[t].df1 = x.df1;
+            hits[t].df2 = x.df2;
+            hits[t].F = (float)x.F;
+            hits[t].F2 = (float)x.F2;
+        }
+        if (F64) F64[t] = x.F;
+        if (F2_64) F2_64[t] = x.F2;
+        if (block) block[t] = blk_of(x);
     }
+    if (n) *n = m;
+}
+
+// ------------------------------------------------------------------ batch --
+
+// Work items of every scan and the queue boundaries of the schedule.
+// Per dataset the items are in locality order: tile-major (super-tiles),
+// phenotype-minor, FAST tiles before GENERAL ones. order 1 puts the GENERAL
+// items of all datasets first, order 2 takes datasets by decreasing Kp, so
+// that the longest items do not end up in the tail. With 8 queues the list
+// is cut into NQ contiguous pieces of equal estimated cost, one per XCD;
+// with 1 queue every workgroup pulls from one shared queue.
+int build_batch_work(epi_batch *b, int form, const epi_schedule &sched)
+{
+    epi_ctx *c = b->ctx;
+    const int nd = (int)b->data.size();
+    std::vector<std::vector<int4>> wk[2];
+    std::vector<std::vector<double>> ct[2];
+    for (int kind = 0; kind < 2; kind++) {
+        wk[kind].resize(nd);
+        ct[kind].resize(nd);
+    }
+    b->pairs = 0;
+    const double unit_fast = form == EPI_FORM_MFMA ? 16.0 : 4.0, unit_gen = form == EPI_FORM_MFMA ? 48.0 : 10.0;
+    for (int di = 0; di < nd; di++) {
+        const Dataset &d = *b->data[di];
+        std::vector<int> sc;
+        for (int s = 0; s < (int)b->scans.size(); s++)
+            if (b->scans[s].first == di) sc.push_back(s);
+        if (sc.empty()) continue;
+        TileLists tl;
+        build_tiles(d, 0, 1 << 30, form, sched, tl);
+        b->pairs += tl.pairs * (long long)sc.size();
+        for (int kind = 0; kind < 2; kind++) {
+            const std::vector<int4> &v = kind ? tl.gen : tl.fast;
+            for (const int4 &t : v) {
+                int ext = (t.y - t.x) | ((t.w - t.z) << 8) | (kind << 16);
+                double cst = (double)(t.y - t.x) * (t.w - t.z) * d.Kp * (kind ? unit_gen : unit_fast) + 5e4;
+                for (int s : sc) {
+                    wk[kind][di].push_back(make_int4(t.x, t.z, ext, s));
+                    ct[kind][di].push_back(cst);
+                }
+            }
+        }
+    }
+    std::vector<int> dorder(nd);
+    for (int i = 0; i < nd; i++) dorder[i] = i;
+    if (sched.order == 2)
+        std::stable_sort(dorder.begin(), dorder.end(),
+                         [&](int x, int y) { return b->data[x]->Kp > b->data[y]->Kp; });
+    std::vector<int4> work;
+    std::vector<double> cost;
+    auto put = [&](int kind, int di) {
+        work.insert(work.end(), wk[kind][di].begin(), wk[kind][di].end());
+        cost.insert(cost.end(), ct[kind][di].begin(), ct[kind][di].end());
+    };
+    if (sched.order == 1) {
+        for (int di : dorder) put(1, di);
+        for (int di : dorder) put(0, di);
+    } else {
+        for (int di : dorder) {
+            put(0, di);
+            put(1, di);
+        }
+    }
+    b->nwork = (long long)work.size();
+    if (work.size() > (size_t)INT32_MAX / 2) return fail(c, EPI_ERR_RANGE, "batch has too many tiles");
+    const int n = (int)work.size();
+    std::vector<int> qbegin(NQ, n), qend(NQ, n);
+    qbegin[0] = 0;
+    if (sched.queues == 1) {
+        qend[0] = n;
+    } else {
+        double total = 0.0;
+        for (double x : cost) total += x;
+        double acc = 0.0;
+        int q = 0;
+        for (int i = 0; i < n && q < NQ - 1; i++) {
+            acc += cost[i];
+            if (acc >= total * (q + 1) / NQ) {
+                qend[q] = i + 1;
+                qbegin[++q] = i + 1;
+            }
+        }
+    }
+    int rc;
+    if (work.size() > b->work_cap) {
+        if ((rc = dalloc(c, b->d_work, work.size()))) return rc;
+        b->work_cap = work.size();
+    }
+    if (!work.empty())
+        HIPCHK(c, hipMemcpyAsync(b->d_work, work.data(), work.size() * sizeof(int4), hipMemcpyHostToDevice,
+                                 c->stream));
+    HIPCHK(c, hipMemcpyAsync(b->d_qbegin, qbegin.data(), NQ * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b->d_qend, qend.data(), NQ * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    // scan descriptors
+    size_t ns = b->scans.size();
+    if (ns > b->scan_cap) {
+        if ((rc = dalloc(c, b->d_mscan, ns)) || (rc = dalloc(c, b->d_pscan, ns))) return rc;
+        b->scan_cap = ns;
+    }
+    std::vector<MfmaScan> ms(ns);
+    std::vector<PlainScan> ps(ns);
+    for (size_t s = 0; s < ns; s++) {
+        const Dataset &d = *b->data[b->scans[s].first];
+        ms[s] = mfma_scan(d, b->scans[s].second, (int)s);
+        ps[s] = plain_scan(d, b->scans[s].second, (int)s);
+    }
+    HIPCHK(c, hipMemcpyAsync(b->d_mscan, ms.data(), ns * sizeof(MfmaScan), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b->d_pscan, ps.data(), ns * sizeof(PlainScan), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    b->work_form = form;
+    b->work_sched = sched;
+    b->dirty = false;
+    return EPI_OK;
+}
+
+bool same_key(const epi_scan_params &x, const epi_scan_params &y)
+{
+    return x.test == y.test && x.form == y.form && x.t1 == y.t1 && x.t2 == y.t2;
+}
+
+void drop_graph(epi_batch *b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->gvalid = false;
+}
+
+const void *batch_kernel(int form, const epi_schedule &sc)
+{
+    if (form == EPI_FORM_MFMA)
+        return sc.fast_rows == 64 ? (const void *)mfma_batch_kernel<64> : (const void *)mfma_batch_kernel<32>;
+    return sc.fast_rows == 32 ? (const void *)plain_batch_kernel<2> : (const void *)plain_batch_kernel<1>;
+}
+
+// Capture queue reset + persistent scan kernel as one graph. The grid is the
+// kernel's occupancy times the CU count: every workgroup is resident and
+// pulls work until the queues run dry.
+int capture_batch(epi_batch *b, const epi_scan_params *p)
+{
+    epi_ctx *c = b->ctx;
+    drop_graph(b);
+    BatchArgs B;
+    B.mscan = b->d_mscan;
+    B.pscan = b->d_pscan;
+    B.work = b->d_work;
+    B.qpos = b->d_qpos;
+    B.qend = b->d_qend;
+    B.hs = make_sink(b->hits, b->nhits, b->cap, p);
+    const void *kfn = batch_kernel(p->form, b->work_sched);
+    int per_cu = 1;
+    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, 256, 0));
+    const int grid = std::max(1, per_cu) * c->ncu;
+    void *args[] = {&B};
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, c->stream, b->d_qpos, (const int *)b->d_qbegin, b->nhits);
+    hipError_t le = hipGetLastError();
+    if (le == hipSuccess) le = hipLaunchKernel(kfn, dim3(grid), dim3(256), args, 0, c->stream);
+    hipError_t ce = hipStreamEndCapture(c->stream, &b->graph);
+    if (le != hipSuccess) return fail(c, EPI_ERR_HIP, "launch during capture: %s", hipGetErrorString(le));
+    if (ce != hipSuccess) return fail(c, EPI_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ce));
+    HIPCHK(c, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->gkey = *p;
+    b->gvalid = true;
+    b->grid = grid;
+    b->captures++;
+    return EPI_OK;
+}
+
+// Work list and graph for (p, the batch's schedule of p->form), rebuilt only
+// when the batch, the form, the schedule or the scan settings changed.
+int batch_prepare(epi_batch *b, const epi_scan_params *p)
+{
+    int rc;
+    const epi_schedule &sc = b->sched[p->form];
+    if (b->dirty || b->work_form != p->form || !same_schedule(b->work_sched, sc)) {
+        drop_graph(b);
+        if ((rc = build_batch_work(b, p->form, sc))) return rc;
+    }
+    if (!b->gvalid || !same_key(b->gkey, *p))
+        if ((rc = capture_batch(b, p))) return rc;
+    return EPI_OK;
+}
+
+// One replay of the captured graph; device time in ms.
+int batch_run(epi_batch *b, double *ms_out)
+{
+    epi_ctx *c = b->ctx;
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, hipGraphLaunch(b->exec, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    unsigned long long nh = 0;
+    HIPCHK(c, hipMemcpyAsync(&nh, b->nhits, sizeof(nh), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    b->last_ms = ms;
+    b->last_nhits = (long long)nh;
+    b->fetched = false;
+    if (ms_out) *ms_out = ms;
+    return EPI_OK;
+}
+
+int batch_fetch(epi_batch *b)
+{
+    if (b->fetched) return EPI_OK;
+    epi_ctx *c = b->ctx;
+    long long have = std::min(b->last_nhits, b->cap);
+    b->hh.resize((size_t)have);
+    if (have) {
+        HIPCHK(c, hipMemcpyAsync(b->hh.data(), b->hits, (size_t)have * sizeof(DevHit), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    b->by_scan.assign(b->scans.size(), {});
+    for (long long t = 0; t < have; t++) {
+        int s = b->hh[t].tag;
+        if (s >= 0 && s < (int)b->scans.size()) b->by_scan[s].push_back((int)t);
+    }
+    b->fetched = true;
+    return EPI_OK;
 }
 
 }  // namespace
@@ -496,7 +888,8 @@ int epi_open(int device, epi_ctx **out)
     if (device < 0 || device >= n) return EPI_ERR_NODEV;
     epi_ctx *c = new epi_ctx();
     c->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+    if (hipSetDevice(device) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
         delete c;
         return EPI_ERR_HIP;
@@ -504,7 +897,10 @@ int epi_open(int device, epi_ctx **out)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
         snprintf(c->name, sizeof(c->name), "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+        c->ncu = prop.multiProcessorCount;
     }
+    if (c->ncu <= 0) c->ncu = 1;
+    for (int f = 0; f < 2; f++) c->sched[f] = default_schedule(f);
     if (dalloc(c, c->nhits, 1) || dalloc(c, c->hits, (size_t)c->cap)) {
         epi_close(c);
         return EPI_ERR_NOMEM;
@@ -518,12 +914,7 @@ void epi_close(epi_ctx *c)
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    dfree(c->words);
-    dfree(c->planes);
-    dfree(c->bits);
-    dfree(c->dmiss);
-    dfree(c->cls);
-    for (auto &p : c->ph) free_phen(p);
+    free_dataset(c->d);
     dfree(c->hits);
     dfree(c->nhits);
     dfree(c->tiles);
@@ -543,175 +934,22 @@ int epi_device_name(const epi_ctx *c, char *buf, size_t n)
 int epi_set_genotypes(epi_ctx *c, const int32_t *words, int32_t nid, int32_t nsnp)
 {
     if (!c) return EPI_ERR_ARG;
-    if (!words || nid < 2 || nsnp < 2) return fail(c, EPI_ERR_ARG, "need nid >= 2, nsnp >= 2 and words");
     HIPCHK(c, hipSetDevice(c->device));
-    if (c->nphen > 0 && nid != c->nid) {  // a new cohort: phenotypes of the old one no longer apply
-        for (auto &p : c->ph) free_phen(p);
-        c->ph.clear();
-        c->nphen = 0;
-    }
-    c->nid = nid;
-    c->nsnp = nsnp;
-    c->npack = nid / 16;
-    c->remain = nid % 16;
-    c->npack1 = c->npack + (c->remain > 0);
-    c->Kp = (nid + KPAD - 1) / KPAD * KPAD;
-    c->Kw = c->Kp / 32;
-    size_t nwords = (size_t)nsnp * c->npack1;
-    int rc;
-    if ((rc = dalloc(c, c->words, nwords)) || (rc = dalloc(c, c->planes, (size_t)nsnp * 3 * c->Kp)) ||
-        (rc = dalloc(c, c->bits, (size_t)3 * c->Kw * nsnp)) || (rc = dalloc(c, c->dmiss, (size_t)nsnp)) ||
-        (rc = dalloc(c, c->cls, (size_t)nsnp)))
-        return rc;
-    HIPCHK(c, hipMemcpyAsync(c->words, words, nwords * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->dmiss, 0, (size_t)nsnp * sizeof(int32_t), c->stream));
-    {
-        long long total = (long long)(c->Kp / 16) * nsnp;
-        int blocks = (int)((total + 255) / 256);
-        hipLaunchKernelGGL(transcode_planes, dim3(blocks), dim3(256), 0, c->stream, (const uint32_t *)c->words, nid,
-                           c->npack, c->remain, c->npack1, nsnp, c->Kp, c->planes, c->dmiss);
-        HIPCHK(c, hipGetLastError());
-        hipLaunchKernelGGL(class_order, dim3((nsnp + 255) / 256), dim3(256), 0, c->stream,
-                           (const uint32_t *)c->words, nid, c->npack, c->remain, c->npack1, nsnp, c->cls);
-        HIPCHK(c, hipGetLastError());
-        hipLaunchKernelGGL(transcode_bits, dim3((nsnp + 255) / 256, c->Kw), dim3(256), 0, c->stream,
-                           (const uint32_t *)c->words, nid, c->npack, c->remain, c->npack1, nsnp, c->Kw, c->cls,
-                           c->bits);
-        HIPCHK(c, hipGetLastError());
-    }
-    c->miss.assign(nsnp, 0);
-    HIPCHK(c, hipMemcpyAsync(c->miss.data(), c->dmiss, (size_t)nsnp * sizeof(int32_t), hipMemcpyDeviceToHost,
-                             c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    c->missc.assign(nsnp + 1, 0);
-    for (int s = 0; s < nsnp; s++) c->missc[s + 1] = c->missc[s] + (c->miss[s] != 0);
-    c->marg_ok = false;
-    // chromosome layout resets to one block over all SNPs
-    c->chrstart.assign(1, 0);
-    c->chrsize.assign(1, nsnp);
-    c->snpchr.assign(nsnp, 0);
     c->last_nhits = 0;
-    return EPI_OK;
+    return ds_set_genotypes(c, c->d, words, nid, nsnp);
 }
 
 int epi_set_phenotypes(epi_ctx *c, const float *phen, int32_t nphen, int32_t nid)
 {
     if (!c) return EPI_ERR_ARG;
-    if (!phen || nphen < 1 || nid < 2) return fail(c, EPI_ERR_ARG, "need phenotypes");
-    if (c->nid && nid != c->nid) return fail(c, EPI_ERR_ARG, "nid %d differs from genotypes' nid %d", nid, c->nid);
     HIPCHK(c, hipSetDevice(c->device));
-    const int Kp = (nid + KPAD - 1) / KPAD * KPAD, Kw = Kp / 32;
-    for (auto &p : c->ph) free_phen(p);
-    c->ph.assign(nphen, PhenDev());
-    c->nphen = nphen;
-    if (!c->nid) {
-        c->nid = nid;
-        c->Kp = Kp;
-        c->Kw = Kw;
-    }
-    std::vector<int8_t> w((size_t)NW * Kp);
-    std::vector<int32_t> qv(Kp);
-    std::vector<float> yc(Kp);
-    std::vector<uint32_t> vb(Kw);
-    for (int r = 0; r < nphen; r++) {
-        const float *y = phen + (size_t)r * nid;
-        PhenDev &p = c->ph[r];
-        double sum = 0.0;
-        int nv = 0;
-        for (int k = 0; k < nid; k++) {
-            float v = y[k];
-            if (v != v || v == -9.0f) continue;
-            if (std::isinf(v)) return fail(c, EPI_ERR_RANGE, "phenotype %d has an infinite value", r);
-            sum += v;
-            nv++;
-        }
-        if (nv < 2) return fail(c, EPI_ERR_RANGE, "phenotype %d has fewer than 2 present values", r);
-        double centre = sum / nv, maxabs = 0.0;
-        for (int k = 0; k < nid; k++) {
-            float v = y[k];
-            if (v != v || v == -9.0f) continue;
-            maxabs = std::max(maxabs, std::fabs((double)v - centre));
-        }
-        int f = 0;
-        if (maxabs > 0.0) f = (int)std::floor(std::log2((double)QMAX / maxabs));
-        while (maxabs > 0.0 && std::ldexp(maxabs, f) > (double)QMAX) f--;
-        std::fill(w.begin(), w.end(), 0);
-        std::fill(qv.begin(), qv.end(), 0);
-        std::fill(yc.begin(), yc.end(), 0.0f);
-        std::fill(vb.begin(), vb.end(), 0u);
-        long long qsum = 0;
-        double sq = 0.0, ys = 0.0, y2 = 0.0;
-        for (int k = 0; k < nid; k++) {
-            float v = y[k];
-            if (v != v || v == -9.0f) continue;
-            double d = (double)v - centre;
-            long long q = std::llround(std::ldexp(d, f));
-            if (q > QMAX || q < -QMAX) return fail(c, EPI_ERR_RANGE, "fixed-point phenotype out of range");
-            long long l0 = ((q % 256) + 256 + 128) % 256 - 128;
-            long long q1 = (q - l0) / 256;
-            long long l1 = ((q1 % 256) + 256 + 128) % 256 - 128;
-            long long l2 = (q1 - l1) / 256;
-            long long z = std::llround((double)q * (double)q / ZSHIFT);
-            long long z0 = ((z % 256) + 256 + 128) % 256 - 128;
-            long long z1q = (z - z0) / 256;
-            long long z1 = ((z1q % 256) + 256 + 128) % 256 - 128;
-            long long z2 = (z1q - z1) / 256;
-            if (l2 < -128 || l2 > 127 || z2 < -128 || z2 > 127) return fail(c, EPI_ERR_RANGE, "limb overflow");
-            w[0 * (size_t)Kp + k] = 1;
-            w[1 * (size_t)Kp + k] = (int8_t)l0;
-            w[2 * (size_t)Kp + k] = (int8_t)l1;
-            w[3 * (size_t)Kp + k] = (int8_t)l2;
-            w[4 * (size_t)Kp + k] = (int8_t)z0;
-            w[5 * (size_t)Kp + k] = (int8_t)z1;
-            w[6 * (size_t)Kp + k] = (int8_t)z2;
-            qv[k] = (int32_t)q;
-            qsum += q;
-            sq += (double)q * (double)q;
-            float ycf = (float)d;
-            yc[k] = ycf;
-            ys += (double)ycf;
-            y2 += (double)ycf * (double)ycf;
-            vb[k >> 5] |= 1u << (k & 31);
-        }
-        p.fbits = f;
-        p.centre = centre;
-        p.nV = nv;
-        p.qV = qsum;
-        p.sqV = sq;
-        p.ysV = ys;
-        p.y2V = y2;
-        int rc;
-        if ((rc = dalloc(c, p.wts, w.size())) || (rc = dalloc(c, p.qv, (size_t)Kp)) ||
-            (rc = dalloc(c, p.yc, (size_t)Kp)) || (rc = dalloc(c, p.vbits, (size_t)Kw)))
-            return rc;
-        HIPCHK(c, hipMemcpyAsync(p.wts, w.data(), w.size(), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(p.qv, qv.data(), (size_t)Kp * 4, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(p.yc, yc.data(), (size_t)Kp * 4, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(p.vbits, vb.data(), (size_t)Kw * 4, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-    }
-    c->marg_ok = false;
-    return EPI_OK;
+    return ds_set_phenotypes(c, c->d, phen, nphen, nid);
 }
 
 int epi_set_chromosomes(epi_ctx *c, const int32_t *chrstart, const int32_t *chrsize, int32_t nchr)
 {
     if (!c) return EPI_ERR_ARG;
-    if (!chrstart || !chrsize || nchr < 1) return fail(c, EPI_ERR_ARG, "need chromosomes");
-    if (!c->nsnp) return fail(c, EPI_ERR_STATE, "set genotypes before chromosomes");
-    std::vector<int> snpchr(c->nsnp, -1);
-    for (int i = 0; i < nchr; i++) {
-        if (chrsize[i] < 0 || chrstart[i] < 0 || (long long)chrstart[i] + chrsize[i] > c->nsnp)
-            return fail(c, EPI_ERR_ARG, "chromosome %d out of range", i);
-        for (int s = chrstart[i]; s < chrstart[i] + chrsize[i]; s++) {
-            if (snpchr[s] >= 0) return fail(c, EPI_ERR_ARG, "chromosomes %d and %d overlap", snpchr[s], i);
-            snpchr[s] = i;
-        }
-    }
-    c->chrstart.assign(chrstart, chrstart + nchr);
-    c->chrsize.assign(chrsize, chrsize + nchr);
-    c->snpchr.swap(snpchr);
-    return EPI_OK;
+    return ds_set_chromosomes(c, c->d, chrstart, chrsize, nchr);
 }
 
 void epi_scan_params_default(epi_scan_params *p)
@@ -744,20 +982,19 @@ int epi_scan(epi_ctx *c, const epi_scan_params *p)
     int rc = check_params(c, p);
     if (rc) return rc;
     HIPCHK(c, hipSetDevice(c->device));
-    if ((rc = ensure_marginals(c))) return rc;
-    int nchr = (int)c->chrstart.size();
+    if ((rc = ds_marginals(c, c->d))) return rc;
+    int nchr = (int)c->d.chrstart.size();
     int nblk = nchr * (nchr + 1) / 2;
     int bb = std::max(0, p->blk_begin), be = p->blk_end <= 0 ? nblk : std::min(nblk, p->blk_end);
-    int TA, TB, split;
-    tile_shape(p->form, TA, TB, split);
     TileLists tl;
-    build_tiles(c, bb, be, TA, TB, split, tl);
+    const epi_schedule &sc = c->sched[p->form];
+    build_tiles(c->d, bb, be, p->form, sc, tl);
     if ((rc = upload_tiles(c, tl))) return rc;
     HIPCHK(c, hipMemsetAsync(c->nhits, 0, sizeof(unsigned long long), c->stream));
     DumpSink ds;
     memset(&ds, 0, sizeof(ds));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    if ((rc = launch_pairs<false>(c, p, tl, ds))) return rc;
+    if ((rc = launch_pairs<false>(c, p, sc, tl, ds))) return rc;
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     unsigned long long nh = 0;
     HIPCHK(c, hipMemcpyAsync(&nh, c->nhits, sizeof(nh), hipMemcpyDeviceToHost, c->stream));
@@ -767,7 +1004,6 @@ int epi_scan(epi_ctx *c, const epi_scan_params *p)
     c->last_ms = ms;
     c->last_pairs = tl.pairs;
     c->last_nhits = (long long)nh;
-    c->last_test = p->test;
     return EPI_OK;
 }
 
@@ -789,36 +1025,26 @@ int epi_get_hits(epi_ctx *c, epi_hit *hits, double *F64, double *F2_64, int32_t 
         HIPCHK(c, hipMemcpyAsync(h.data(), c->hits, (size_t)have * sizeof(DevHit), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
-    auto blk_of = [&](const DevHit &x) {
-        int i = c->snpchr[x.snp1], j = c->snpchr[x.snp2];
-        return i * (i + 1) / 2 + j;
-    };
-    std::vector<long long> key((size_t)have);
-    for (long long t = 0; t < have; t++) key[t] = t;
-    std::sort(key.begin(), key.end(), [&](long long x, long long y) {
-        const DevHit &p = h[x], &q = h[y];
-        int bp = blk_of(p), bq = blk_of(q);
-        if (bp != bq) return bp < bq;
-        if (p.snp1 != q.snp1) return p.snp1 < q.snp1;
-        if (p.snp2 != q.snp2) return p.snp2 < q.snp2;
-        return p.phen < q.phen;
-    });
-    long long m = std::min<long long>(have, cap);
-    for (long long t = 0; t < m; t++) {
-        const DevHit &x = h[key[t]];
-        if (hits) {
-            hits[t].snp1 = x.snp1;
-            hits[t].snp2 = x.snp2;
-            hits[t].df1 = x.df1;
-            hits[t].df2 = x.df2;
-            hits[t].F = (float)x.F;
-            hits[t].F2 = (float)x.F2;
-        }
-        if (F64) F64[t] = x.F;
-        if (F2_64) F2_64[t] = x.F2;
-        if (block) block[t] = blk_of(x);
-    }
+    std::vector<int> idx((size_t)have);
+    for (long long t = 0; t < have; t++) idx[t] = (int)t;
+    long long m = 0;
+    emit_hits(c->d, h, idx, hits, F64, F2_64, block, cap, &m);
     if (n) *n = m;
+    return EPI_OK;
+}
+
+int epi_set_schedule(epi_ctx *c, int32_t form, const epi_schedule *s)
+{
+    if (!c || !s || (form != EPI_FORM_PLAIN && form != EPI_FORM_MFMA)) return EPI_ERR_ARG;
+    if (!valid_schedule(form, *s)) return fail(c, EPI_ERR_ARG, "invalid schedule for this form");
+    c->sched[form] = *s;
+    return EPI_OK;
+}
+
+int epi_get_schedule(epi_ctx *c, int32_t form, epi_schedule *s)
+{
+    if (!c || !s || (form != EPI_FORM_PLAIN && form != EPI_FORM_MFMA)) return EPI_ERR_ARG;
+    *s = c->sched[form];
     return EPI_OK;
 }
 
@@ -837,33 +1063,17 @@ int epi_pair_dump(epi_ctx *c, const epi_scan_params *p, int32_t a0, int32_t a1, 
     if (!c) return EPI_ERR_ARG;
     int rc = check_params(c, p);
     if (rc) return rc;
-    if (a0 < 0 || b0 < 0 || a1 > c->nsnp || b1 > c->nsnp || a1 <= a0 || b1 <= b0)
+    if (a0 < 0 || b0 < 0 || a1 > c->d.nsnp || b1 > c->d.nsnp || a1 <= a0 || b1 <= b0)
         return fail(c, EPI_ERR_ARG, "dump rectangle out of range");
     HIPCHK(c, hipSetDevice(c->device));
-    if ((rc = ensure_marginals(c))) return rc;
-    int TA, TB, split;
-    tile_shape(p->form, TA, TB, split);
+    if ((rc = ds_marginals(c, c->d))) return rc;
+    const epi_schedule &sc = c->sched[p->form];
+    const int TA = sc.fast_rows, TB = TILE_B;
     // Tiles over the rectangle in global SNP space (no block restriction).
     TileLists all;
-    {
-        // emit tiles covering [a0,a1) x [b0,b1) with the same FAST/GENERAL split
-        auto anymiss = [&](int s0, int s1) { return c->missc[s1] - c->missc[s0] > 0; };
-        for (int x = a0; x < a1; x += TA)
-            for (int y = b0; y < b1; y += TB) {
-                int x1 = std::min(a1, x + TA), y1 = std::min(b1, y + TB);
-                if (x1 - 1 <= y) continue;
-                if (!anymiss(x, x1) && !anymiss(y, y1)) {
-                    all.fast.push_back(make_int4(x, x1, y, y1));
-                } else {
-                    int sub = TA / split;
-                    for (int u = x; u < x1; u += sub) {
-                        int u1 = std::min(x1, u + sub);
-                        if (u1 - 1 <= y) continue;
-                        all.gen.push_back(make_int4(u, u1, y, y1));
-                    }
-                }
-            }
-    }
+    for (int x = a0; x < a1; x += TA)
+        for (int y = b0; y < b1; y += TB)
+            emit_tile(c->d, x, std::min(a1, x + TA), y, std::min(b1, y + TB), gen_cols(p->form, sc), all);
     if ((rc = upload_tiles(c, all))) return rc;
     long long np = (long long)(a1 - a0) * (b1 - b0);
     int32_t *dc = nullptr;
@@ -884,7 +1094,7 @@ int epi_pair_dump(epi_ctx *c, const epi_scan_params *p, int32_t a0, int32_t a1, 
     ds.a1 = a1;
     ds.b0 = b0;
     ds.b1 = b1;
-    PhenDev &ph = c->ph[p->phen];
+    const PhenDev &ph = c->d.ph[p->phen];
     ds.unit = p->form == EPI_FORM_MFMA ? std::ldexp(1.0, -ph.fbits) : 1.0;
     ds.centre = ph.centre;
     // Pre-fill with the caller's buffers so that pairs with a <= b keep their values.
@@ -897,7 +1107,7 @@ int epi_pair_dump(epi_ctx *c, const epi_scan_params *p, int32_t a0, int32_t a1, 
     h2d(dsum, sums, (size_t)np * 9 * 8);
     h2d(dF, F, (size_t)np * 8);
     h2d(dF2, F2, (size_t)np * 8);
-    if (hrc == EPI_OK) hrc = launch_pairs<true>(c, p, all, ds);
+    if (hrc == EPI_OK) hrc = launch_pairs<true>(c, p, sc, all, ds);
     auto d2h = [&](void *hdst, const void *d, size_t bytes) {
         if (hrc == EPI_OK && hdst && hipMemcpyAsync(hdst, d, bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
             hrc = fail(c, EPI_ERR_HIP, "dump copy back failed");
@@ -912,6 +1122,258 @@ int epi_pair_dump(epi_ctx *c, const epi_scan_params *p, int32_t a0, int32_t a1, 
     dfree(dF);
     dfree(dF2);
     return hrc;
+}
+
+// ------------------------------------------------------------------ batch --
+
+int epi_batch_create(epi_ctx *c, epi_batch **out)
+{
+    if (!c || !out) return EPI_ERR_ARG;
+    *out = nullptr;
+    HIPCHK(c, hipSetDevice(c->device));
+    epi_batch *b = new epi_batch();
+    b->ctx = c;
+    for (int f = 0; f < 2; f++) b->sched[f] = default_schedule(f);
+    int rc;
+    if ((rc = dalloc(c, b->d_qpos, NQ)) || (rc = dalloc(c, b->d_qbegin, NQ)) || (rc = dalloc(c, b->d_qend, NQ)) ||
+        (rc = dalloc(c, b->nhits, 1)) || (rc = dalloc(c, b->hits, (size_t)b->cap))) {
+        epi_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return EPI_OK;
+}
+
+void epi_batch_destroy(epi_batch *b)
+{
+    if (!b) return;
+    epi_ctx *c = b->ctx;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    drop_graph(b);
+    for (Dataset *d : b->data) {
+        free_dataset(*d);
+        delete d;
+    }
+    dfree(b->d_mscan);
+    dfree(b->d_pscan);
+    dfree(b->d_work);
+    dfree(b->d_qpos);
+    dfree(b->d_qbegin);
+    dfree(b->d_qend);
+    dfree(b->hits);
+    dfree(b->nhits);
+    delete b;
+}
+
+int epi_batch_add_dataset(epi_batch *b, const int32_t *words, int32_t nid, int32_t nsnp, const int32_t *chrstart,
+                          const int32_t *chrsize, int32_t nchr, int32_t *dataset)
+{
+    if (!b) return EPI_ERR_ARG;
+    epi_ctx *c = b->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    Dataset *d = new Dataset();
+    int rc = ds_set_genotypes(c, *d, words, nid, nsnp);
+    if (!rc && chrstart) rc = ds_set_chromosomes(c, *d, chrstart, chrsize, nchr);
+    if (rc) {
+        free_dataset(*d);
+        delete d;
+        return rc;
+    }
+    b->data.push_back(d);
+    b->dirty = true;
+    if (dataset) *dataset = (int32_t)b->data.size() - 1;
+    return EPI_OK;
+}
+
+int epi_batch_add_phenotypes(epi_batch *b, int32_t dataset, const float *phen, int32_t nphen, int32_t *first_scan)
+{
+    if (!b) return EPI_ERR_ARG;
+    epi_ctx *c = b->ctx;
+    if (dataset < 0 || dataset >= (int)b->data.size()) return fail(c, EPI_ERR_ARG, "no such dataset");
+    Dataset &d = *b->data[dataset];
+    if (!d.ph.empty()) return fail(c, EPI_ERR_STATE, "dataset %d already has phenotypes", dataset);
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = ds_set_phenotypes(c, d, phen, nphen, d.nid);
+    if (!rc) rc = ds_marginals(c, d);
+    if (rc) return rc;
+    if (first_scan) *first_scan = (int32_t)b->scans.size();
+    for (int r = 0; r < nphen; r++) b->scans.push_back({dataset, r});
+    b->dirty = true;
+    return EPI_OK;
+}
+
+int epi_batch_set_hit_capacity(epi_batch *b, int64_t cap)
+{
+    if (!b || cap < 1) return EPI_ERR_ARG;
+    epi_ctx *c = b->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = dalloc(c, b->hits, (size_t)cap);
+    if (rc) return rc;
+    b->cap = cap;
+    b->last_nhits = 0;
+    b->fetched = false;
+    drop_graph(b);
+    return EPI_OK;
+}
+
+int epi_batch_scan(epi_batch *b, const epi_scan_params *p)
+{
+    if (!b) return EPI_ERR_ARG;
+    epi_ctx *c = b->ctx;
+    int rc = check_common(c, p);
+    if (rc) return rc;
+    if (b->scans.empty()) return fail(c, EPI_ERR_STATE, "batch has no scans");
+    HIPCHK(c, hipSetDevice(c->device));
+    if ((rc = batch_prepare(b, p))) return rc;
+    return batch_run(b, nullptr);
+}
+
+int epi_batch_set_schedule(epi_batch *b, int32_t form, const epi_schedule *s)
+{
+    if (!b || !s || (form != EPI_FORM_PLAIN && form != EPI_FORM_MFMA)) return EPI_ERR_ARG;
+    if (!valid_schedule(form, *s)) return fail(b->ctx, EPI_ERR_ARG, "invalid schedule for this form");
+    b->sched[form] = *s;
+    return EPI_OK;
+}
+
+int epi_batch_get_schedule(epi_batch *b, int32_t form, epi_schedule *s)
+{
+    if (!b || !s || (form != EPI_FORM_PLAIN && form != EPI_FORM_MFMA)) return EPI_ERR_ARG;
+    *s = b->sched[form];
+    return EPI_OK;
+}
+
+int epi_batch_autotune(epi_batch *b, const epi_scan_params *p, int32_t dims, int32_t reps, epi_schedule *best,
+                       double *best_ms, epi_tune_entry *log, int32_t log_cap, int32_t *nlog)
+{
+    if (!b) return EPI_ERR_ARG;
+    epi_ctx *c = b->ctx;
+    int rc = check_common(c, p);
+    if (rc) return rc;
+    if (b->scans.empty()) return fail(c, EPI_ERR_STATE, "batch has no scans");
+    if (reps < 1 || reps > 100) return fail(c, EPI_ERR_ARG, "reps must be in [1, 100]");
+    if (dims == 0) dims = EPI_TUNE_ALL;
+    HIPCHK(c, hipSetDevice(c->device));
+    const int form = p->form;
+    int n = 0;
+    std::vector<std::pair<epi_schedule, double>> seen;
+    // median device time of reps replays after one warm-up replay
+    auto measure = [&](const epi_schedule &s, double &ms) -> int {
+        for (auto &e : seen)
+            if (same_schedule(e.first, s)) {
+                ms = e.second;
+                return EPI_OK;
+            }
+        b->sched[form] = s;
+        int r = batch_prepare(b, p);
+        if (r) return r;
+        if ((r = batch_run(b, nullptr))) return r;
+        std::vector<double> t(reps);
+        for (int i = 0; i < reps; i++)
+            if ((r = batch_run(b, &t[i]))) return r;
+        std::sort(t.begin(), t.end());
+        ms = t[reps / 2];
+        seen.push_back({s, ms});
+        if (log && n < log_cap) {
+            log[n].sched = s;
+            log[n].ms = ms;
+        }
+        n++;
+        return EPI_OK;
+    };
+    epi_schedule cur = b->sched[form];
+    double tcur = 0.0;
+    if ((rc = measure(cur, tcur))) return rc;
+    // coordinate descent: each dimension in turn, keeping the best value
+    const int rows_m[] = {64, 32}, rows_p[] = {32, 16};
+    const int sup[][2] = {{8, 4}, {4, 4}, {16, 2}, {4, 8}, {1, 1}};
+    const int queues[] = {8, 1};
+    const int orders[] = {0, 1, 2};
+    auto tryit = [&](const epi_schedule &s) -> int {
+        double t = 0.0;
+        int r = measure(s, t);
+        if (r) return r;
+        if (t < tcur) {
+            cur = s;
+            tcur = t;
+        }
+        return EPI_OK;
+    };
+    if (dims & EPI_TUNE_ROWS)
+        for (int v : (form == EPI_FORM_MFMA ? rows_m : rows_p)) {
+            epi_schedule s = cur;
+            s.fast_rows = v;
+            if ((rc = tryit(s))) return rc;
+        }
+    if (dims & EPI_TUNE_SUPER)
+        for (auto &v : sup) {
+            epi_schedule s = cur;
+            s.super_a = v[0];
+            s.super_b = v[1];
+            if ((rc = tryit(s))) return rc;
+        }
+    if (dims & EPI_TUNE_QUEUES)
+        for (int v : queues) {
+            epi_schedule s = cur;
+            s.queues = v;
+            if ((rc = tryit(s))) return rc;
+        }
+    if (dims & EPI_TUNE_ORDER)
+        for (int v : orders) {
+            epi_schedule s = cur;
+            s.order = v;
+            if ((rc = tryit(s))) return rc;
+        }
+    b->sched[form] = cur;
+    if (best) *best = cur;
+    if (best_ms) *best_ms = tcur;
+    if (nlog) *nlog = n;
+    return EPI_OK;
+}
+
+int epi_batch_hit_count(epi_batch *b, int32_t scan, int64_t *nhits, int32_t *overflow)
+{
+    if (!b) return EPI_ERR_ARG;
+    epi_ctx *c = b->ctx;
+    if (scan < -1 || scan >= (int)b->scans.size()) return fail(c, EPI_ERR_ARG, "no such scan");
+    if (overflow) *overflow = b->last_nhits > b->cap;
+    if (scan < 0) {
+        if (nhits) *nhits = b->last_nhits;
+        return EPI_OK;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = batch_fetch(b);
+    if (rc) return rc;
+    if (nhits) *nhits = (int64_t)b->by_scan[scan].size();
+    return EPI_OK;
+}
+
+int epi_batch_get_hits(epi_batch *b, int32_t scan, epi_hit *hits, double *F64, double *F2_64, int32_t *block,
+                       int64_t cap, int64_t *n)
+{
+    if (!b || cap < 0) return EPI_ERR_ARG;
+    epi_ctx *c = b->ctx;
+    if (scan < 0 || scan >= (int)b->scans.size()) return fail(c, EPI_ERR_ARG, "no such scan");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = batch_fetch(b);
+    if (rc) return rc;
+    long long m = 0;
+    emit_hits(*b->data[b->scans[scan].first], b->hh, b->by_scan[scan], hits, F64, F2_64, block, cap, &m);
+    if (n) *n = m;
+    return EPI_OK;
+}
+
+int epi_batch_stats(epi_batch *b, double *kernel_ms, int64_t *pairs, int64_t *tiles, int32_t *captures,
+                    int32_t *grid)
+{
+    if (!b) return EPI_ERR_ARG;
+    if (grid) *grid = b->grid;
+    if (kernel_ms) *kernel_ms = b->last_ms;
+    if (pairs) *pairs = b->pairs;
+    if (tiles) *tiles = b->nwork;
+    if (captures) *captures = b->captures;
+    return EPI_OK;
 }
 
 }  // extern "C"

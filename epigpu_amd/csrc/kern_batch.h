@@ -1,0 +1,108 @@
+// Persistent batched scan: one launch walks the tiles of many scans (datasets
+// of uneven N and M, and several phenotypes per dataset).
+//
+// The host orders the work items so that consecutive items share SNP data
+// (tile-major, phenotype-minor) and cuts the list into NQ contiguous queues of
+// equal estimated cost, one per XCD. A workgroup pulls items from the queue of
+// its XCD (workgroups are dealt to XCDs round-robin, so blockIdx % 8 is its
+// XCD) and, when that queue is empty, from the others in turn. Every pull is
+// one atomicAdd; a pull past the end of every queue ends the workgroup, so all
+// workgroups exit once the items run out. The queue heads are reset by a
+// kernel (not a memset) so that the reset and the scan can be captured in one
+// hipGraph and replayed.
+#pragma once
+
+#include "kern_mfma.h"
+#include "kern_plain.h"
+
+namespace epi {
+
+constexpr int NQ = 8;  // work queues, one per XCD
+
+struct BatchArgs {
+    const MfmaScan *mscan;   // [nscan]
+    const PlainScan *pscan;  // [nscan]
+    const int4 *work;        // {a0, b0, alen | blen << 8 | general << 16, scan}
+    int *qpos;               // [NQ] next item of each queue (reset to qbegin)
+    const int *qend;         // [NQ] one past the last item of each queue
+    HitSink hs;
+};
+
+__global__ void batch_reset(int *qpos, const int *qbegin, unsigned long long *nhits)
+{
+    const int t = threadIdx.x;
+    if (t < NQ) qpos[t] = qbegin[t];
+    if (t == 0) *nhits = 0ull;
+}
+
+// Called by one thread: the next item for a workgroup whose home queue is
+// `home`, or -1 when every queue is exhausted.
+__device__ __forceinline__ int next_item(const BatchArgs &B, int home)
+{
+    for (int k = 0; k < NQ; k++) {
+        const int q = (home + k) % NQ;
+        if (__hip_atomic_load(&B.qpos[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= B.qend[q]) continue;
+        const int v = atomicAdd(&B.qpos[q], 1);
+        if (v < B.qend[q]) return v;
+    }
+    return -1;
+}
+
+// TA: FAST tiles are TA x 64 (TA = 64 or 32), GENERAL tiles 16 x TA.
+template <int TA>
+__global__ __launch_bounds__(256, 64 / TA) void mfma_batch_kernel(const BatchArgs B)
+{
+    constexpr int NBG = TA / 32;
+    constexpr int LDS = fast_lds<TA>() > general_lds<NBG>() ? fast_lds<TA>() : general_lds<NBG>();
+    __shared__ __attribute__((aligned(16))) uint8_t lds[LDS];
+    __shared__ int item;
+    const int home = blockIdx.x % NQ;
+    const DumpSink nodump = {};
+    for (;;) {
+        __syncthreads();  // the previous tile is done with LDS and `item`
+        if (threadIdx.x == 0) item = next_item(B, home);
+        __syncthreads();
+        const int it = item;
+        if (it < 0) break;
+        const int4 w = B.work[it];
+        const int a0 = __builtin_amdgcn_readfirstlane(w.x), b0 = __builtin_amdgcn_readfirstlane(w.y);
+        const int ext = __builtin_amdgcn_readfirstlane(w.z), sc = __builtin_amdgcn_readfirstlane(w.w);
+        const MfmaScan S = B.mscan[sc];
+        HitSink hs = B.hs;
+        hs.tag = sc;
+        const int a1 = a0 + (ext & 255), b1 = b0 + ((ext >> 8) & 255);
+        if ((ext >> 16) == 0)
+            mfma_fast_tile<TA, false>(S, hs, nodump, a0, a1, b0, b1, lds);
+        else
+            mfma_general_tile<NBG, false>(S, hs, nodump, a0, a1, b0, b1, lds);
+    }
+}
+
+// PA: FAST tiles are 16 PA x 64 (PA = 2 or 1); GENERAL tiles are 16 x 64.
+template <int PA>
+__global__ __launch_bounds__(256) void plain_batch_kernel(const BatchArgs B)
+{
+    __shared__ int item;
+    const int home = blockIdx.x % NQ;
+    const DumpSink nodump = {};
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) item = next_item(B, home);
+        __syncthreads();
+        const int it = item;
+        if (it < 0) break;
+        const int4 w = B.work[it];
+        const int a0 = __builtin_amdgcn_readfirstlane(w.x), b0 = __builtin_amdgcn_readfirstlane(w.y);
+        const int ext = __builtin_amdgcn_readfirstlane(w.z), sc = __builtin_amdgcn_readfirstlane(w.w);
+        const PlainScan S = B.pscan[sc];
+        HitSink hs = B.hs;
+        hs.tag = sc;
+        const int a1 = a0 + (ext & 255), b1 = b0 + ((ext >> 8) & 255);
+        if ((ext >> 16) == 0)
+            plain_tile<PA, 4, true, false>(S, hs, nodump, a0, a1, b0, b1);
+        else
+            plain_tile<1, 4, false, false>(S, hs, nodump, a0, a1, b0, b1);
+    }
+}
+
+}  // namespace epi

@@ -22,24 +22,30 @@ namespace epi {
 
 constexpr int KW = 8;  // words (of 32 individuals) per LDS stage
 
-struct PlainArgs {
+struct PlainScan {          // one dataset x one phenotype
     const uint32_t *bits;   // [3][Kw][nsnp]
     const float *yc;        // [Kp] centred FP32 phenotype, 0 where missing
     const uint32_t *vbits;  // [Kw] phenotype-present bits
     const Marg *marg;       // [nsnp]
-    const int4 *tiles;
-    int ntiles;
     int nsnp;
     int Kw;
     int nV;
+    int tag;                // written into the hits
     double ysV;             // sum of yc over present
     double y2V;             // sum of yc^2 over present
+};
+
+struct PlainArgs {
+    PlainScan S;
+    const int4 *tiles;
+    int ntiles;
     HitSink hs;
     DumpSink ds;
 };
 
 template <int PA, int PB, bool FAST, bool DUMP>
-__global__ __launch_bounds__(256) void plain_kernel(const PlainArgs P)
+__device__ __forceinline__ void plain_tile(const PlainScan &P, const HitSink &hs, const DumpSink &dsk, int a0, int a1,
+                                           int b0, int b1)
 {
     constexpr int NP = FAST ? 2 : 3;
     constexpr int TA = 16 * PA, TB = 16 * PB;
@@ -49,8 +55,6 @@ __global__ __launch_bounds__(256) void plain_kernel(const PlainArgs P)
     __shared__ uint32_t sv[KW];
 
     const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-    const int4 T = P.tiles[xcd_tile(blockIdx.x, P.ntiles)];
-    const int a0 = T.x, a1 = T.y, b0 = T.z, b1 = T.w;
     const int nsnp = P.nsnp;
 
     int cnt[PA][PB][NP][NP];
@@ -211,12 +215,29 @@ __global__ __launch_bounds__(256) void plain_kernel(const PlainArgs P)
             }
             int n[9];
             double s[9];
+            float sf[9];
             cells_from_x(xc, n);
             cells_from_x(xs, s);
+#pragma unroll
+            for (int c = 0; c < 9; c++) sf[c] = (float)s[c];
+            auto exact = [&](double sd[9], double &sqd) {
+#pragma unroll
+                for (int c = 0; c < 9; c++) sd[c] = s[c];
+                sqd = sq;
+            };
             const bool valid = a < a1 && b < b1 && a > b;
-            finish_pair<DUMP>(P.hs, P.ds, valid, a, b, n, s, sq, ma.cls, mb.cls);
+            finish_pair<DUMP>(hs, dsk, valid, a, b, n, sf, (float)sq, exact, ma.cls, mb.cls);
         }
     }
+}
+
+template <int PA, int PB, bool FAST, bool DUMP>
+__global__ __launch_bounds__(256) void plain_kernel(const PlainArgs A)
+{
+    const int4 T = A.tiles[xcd_tile(blockIdx.x, A.ntiles)];
+    HitSink hs = A.hs;
+    hs.tag = A.S.tag;
+    plain_tile<PA, PB, FAST, DUMP>(A.S, hs, A.ds, T.x, T.y, T.z, T.w);
 }
 
 }  // namespace epi

@@ -5,6 +5,7 @@ module only moves numpy arrays across their C ABI. Loading fails loudly when a
 library has not been built: there is no Python fallback for any computation.
 """
 import ctypes
+import weakref
 import os
 
 import numpy as np
@@ -38,6 +39,21 @@ class ScanParams(ctypes.Structure):
     _fields_ = [("test", ctypes.c_int32), ("form", ctypes.c_int32), ("t1", ctypes.c_float),
                 ("t2", ctypes.c_float), ("phen", ctypes.c_int32), ("blk_begin", ctypes.c_int32),
                 ("blk_end", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4)]
+
+
+class Schedule(ctypes.Structure):
+    _fields_ = [("fast_rows", ctypes.c_int32), ("super_a", ctypes.c_int32), ("super_b", ctypes.c_int32),
+                ("queues", ctypes.c_int32), ("order", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k in ("fast_rows", "super_a", "super_b", "queues", "order")}
+
+
+class TuneEntry(ctypes.Structure):
+    _fields_ = [("sched", Schedule), ("ms", ctypes.c_double)]
+
+
+TUNE_ROWS, TUNE_SUPER, TUNE_QUEUES, TUNE_ORDER, TUNE_ALL = 1, 2, 4, 8, 15
 
 
 class RunOpts(ctypes.Structure):
@@ -110,6 +126,21 @@ def _declare(name, L):
             "epi_set_hit_capacity": (ctypes.c_int, [vp, i64]),
             "epi_last_scan_stats": (ctypes.c_int, [vp, _p(f64), _p(i64), _p(i32)]),
             "epi_pair_dump": (ctypes.c_int, [vp, _p(ScanParams), i32, i32, i32, i32, vp, vp, vp, vp]),
+            "epi_batch_create": (ctypes.c_int, [vp, _p(vp)]),
+            "epi_batch_destroy": (None, [vp]),
+            "epi_batch_add_dataset": (ctypes.c_int, [vp, vp, i32, i32, vp, vp, i32, _p(i32)]),
+            "epi_batch_add_phenotypes": (ctypes.c_int, [vp, i32, vp, i32, _p(i32)]),
+            "epi_batch_set_hit_capacity": (ctypes.c_int, [vp, i64]),
+            "epi_batch_scan": (ctypes.c_int, [vp, _p(ScanParams)]),
+            "epi_batch_hit_count": (ctypes.c_int, [vp, i32, _p(i64), _p(i32)]),
+            "epi_batch_get_hits": (ctypes.c_int, [vp, i32, vp, vp, vp, vp, i64, _p(i64)]),
+            "epi_batch_stats": (ctypes.c_int, [vp, _p(f64), _p(i64), _p(i64), _p(i32), _p(i32)]),
+            "epi_set_schedule": (ctypes.c_int, [vp, i32, _p(Schedule)]),
+            "epi_get_schedule": (ctypes.c_int, [vp, i32, _p(Schedule)]),
+            "epi_batch_set_schedule": (ctypes.c_int, [vp, i32, _p(Schedule)]),
+            "epi_batch_get_schedule": (ctypes.c_int, [vp, i32, _p(Schedule)]),
+            "epi_batch_autotune": (ctypes.c_int, [vp, _p(ScanParams), i32, i32, _p(Schedule), _p(f64),
+                                                  _p(TuneEntry), i32, _p(i32)]),
         }
     else:
         sig = {
@@ -316,12 +347,15 @@ class Scanner:
         if rc:
             raise EpiError(rc, "epi_open: " + self.L.epi_strerror(rc).decode())
         self.nid = None
+        self._batches = weakref.WeakSet()  # a batch must be destroyed before its context
 
     def _chk(self, rc, what):
         if rc:
             raise EpiError(rc, f"{what}: {self.L.epi_last_error(self.ctx).decode()}")
 
     def close(self):
+        for b in list(self._batches):
+            b.close()
         if self.ctx:
             self.L.epi_close(self.ctx)
             self.ctx = ctypes.c_void_p()
@@ -361,6 +395,18 @@ class Scanner:
 
     def set_hit_capacity(self, cap):
         self._chk(self.L.epi_set_hit_capacity(self.ctx, cap), "epi_set_hit_capacity")
+
+    def schedule(self, form="mfma"):
+        sc = Schedule()
+        self._chk(self.L.epi_get_schedule(self.ctx, FORMS[form], ctypes.byref(sc)), "epi_get_schedule")
+        return sc.as_dict()
+
+    def set_schedule(self, form="mfma", **kw):
+        sc = Schedule()
+        self._chk(self.L.epi_get_schedule(self.ctx, FORMS[form], ctypes.byref(sc)), "epi_get_schedule")
+        for k, v in kw.items():
+            setattr(sc, k, v)
+        self._chk(self.L.epi_set_schedule(self.ctx, FORMS[form], ctypes.byref(sc)), "epi_set_schedule")
 
     @staticmethod
     def params(test="f", form="mfma", t1=6.5, t2=10.5, phen=0, blk_begin=0, blk_end=0):
@@ -402,3 +448,117 @@ class Scanner:
         self._chk(self.L.epi_pair_dump(self.ctx, ctypes.byref(p), a0, a1, b0, b1, _ptr(counts), _ptr(sums),
                                        _ptr(F), _ptr(F2)), "epi_pair_dump")
         return counts, sums, F, F2
+
+
+class Batch:
+    """An episcan.h batch: many (dataset, phenotype) scans in one captured persistent launch.
+
+    Holds a reference to its Scanner, whose context must outlive it."""
+
+    def __init__(self, scanner):
+        self.sc = scanner
+        self.L = scanner.L
+        self.b = ctypes.c_void_p()
+        scanner._chk(self.L.epi_batch_create(scanner.ctx, ctypes.byref(self.b)), "epi_batch_create")
+        scanner._batches.add(self)
+        self.nscan = 0
+
+    def _chk(self, rc, what):
+        self.sc._chk(rc, what)
+
+    def close(self):
+        if self.b:
+            self.L.epi_batch_destroy(self.b)
+            self.b = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_dataset(self, words, nid, chr_starts=None, chr_sizes=None):
+        words = np.ascontiguousarray(words, dtype=np.int32)
+        assert words.shape[1] == layout(nid)[2]
+        d = ctypes.c_int32()
+        if chr_starts is None:
+            s = z = None
+            nchr = 0
+        else:
+            s = np.ascontiguousarray(chr_starts, dtype=np.int32)
+            z = np.ascontiguousarray(chr_sizes, dtype=np.int32)
+            nchr = len(s)
+        self._chk(self.L.epi_batch_add_dataset(self.b, _ptr(words), nid, words.shape[0], _ptr(s), _ptr(z), nchr,
+                                               ctypes.byref(d)), "epi_batch_add_dataset")
+        return d.value
+
+    def add_phenotypes(self, dataset, phen):
+        phen = np.ascontiguousarray(np.atleast_2d(phen), dtype=np.float32)
+        first = ctypes.c_int32()
+        self._chk(self.L.epi_batch_add_phenotypes(self.b, dataset, _ptr(phen), phen.shape[0], ctypes.byref(first)),
+                  "epi_batch_add_phenotypes")
+        self.nscan = first.value + phen.shape[0]
+        return list(range(first.value, first.value + phen.shape[0]))
+
+    def set_hit_capacity(self, cap):
+        self._chk(self.L.epi_batch_set_hit_capacity(self.b, cap), "epi_batch_set_hit_capacity")
+
+    def scan(self, test="f", form="mfma", t1=6.5, t2=10.5):
+        p = Scanner.params(test, form, t1, t2)
+        self._chk(self.L.epi_batch_scan(self.b, ctypes.byref(p)), "epi_batch_scan")
+        return self.stats()
+
+    def stats(self):
+        ms, pairs, tiles = ctypes.c_double(), ctypes.c_int64(), ctypes.c_int64()
+        cap, grid = ctypes.c_int32(), ctypes.c_int32()
+        self._chk(self.L.epi_batch_stats(self.b, ctypes.byref(ms), ctypes.byref(pairs), ctypes.byref(tiles),
+                                         ctypes.byref(cap), ctypes.byref(grid)), "epi_batch_stats")
+        n, ovf = self.hit_count(-1)
+        return {"kernel_ms": ms.value, "pairs": pairs.value, "tiles": tiles.value, "captures": cap.value,
+                "grid": grid.value, "nhits": n, "overflow": ovf}
+
+    def schedule(self, form="mfma"):
+        sc = Schedule()
+        self._chk(self.L.epi_batch_get_schedule(self.b, FORMS[form], ctypes.byref(sc)), "epi_batch_get_schedule")
+        return sc.as_dict()
+
+    def set_schedule(self, form="mfma", **kw):
+        sc = Schedule()
+        self._chk(self.L.epi_batch_get_schedule(self.b, FORMS[form], ctypes.byref(sc)), "epi_batch_get_schedule")
+        for k, v in kw.items():
+            setattr(sc, k, v)
+        self._chk(self.L.epi_batch_set_schedule(self.b, FORMS[form], ctypes.byref(sc)), "epi_batch_set_schedule")
+
+    def autotune(self, test="f", form="mfma", t1=6.5, t2=10.5, dims=TUNE_ALL, reps=1):
+        """Measure candidate schedules (coordinate descent) and keep the fastest."""
+        p = Scanner.params(test, form, t1, t2)
+        best, ms, n = Schedule(), ctypes.c_double(), ctypes.c_int32()
+        log = (TuneEntry * 64)()
+        self._chk(self.L.epi_batch_autotune(self.b, ctypes.byref(p), dims, reps, ctypes.byref(best),
+                                            ctypes.byref(ms), log, 64, ctypes.byref(n)), "epi_batch_autotune")
+        return {"best": best.as_dict(), "best_ms": ms.value,
+                "log": [dict(log[i].sched.as_dict(), ms=log[i].ms) for i in range(min(n.value, 64))]}
+
+    def hit_count(self, scan=-1):
+        n, ovf = ctypes.c_int64(), ctypes.c_int32()
+        self._chk(self.L.epi_batch_hit_count(self.b, scan, ctypes.byref(n), ctypes.byref(ovf)),
+                  "epi_batch_hit_count")
+        return n.value, bool(ovf.value)
+
+    def hits(self, scan):
+        m, _ = self.hit_count(scan)
+        hits = np.zeros(m, dtype=HIT_DTYPE)
+        F = np.zeros(m, dtype=np.float64)
+        F2 = np.zeros(m, dtype=np.float64)
+        blk = np.zeros(m, dtype=np.int32)
+        got = ctypes.c_int64()
+        self._chk(self.L.epi_batch_get_hits(self.b, scan, _ptr(hits), _ptr(F), _ptr(F2), _ptr(blk), m,
+                                            ctypes.byref(got)), "epi_batch_get_hits")
+        g = got.value
+        return {"nhits": g, "hits": hits[:g], "F64": F[:g], "F2_64": F2[:g], "block": blk[:g]}

@@ -108,6 +108,25 @@ int epi_get_hits(epi_ctx *ctx, epi_hit *hits, double *F64, double *F2_64, int32_
 /* Device capacity of the hit buffer (records); default 1<<22. */
 int epi_set_hit_capacity(epi_ctx *ctx, int64_t cap);
 
+/* Tile shape and work order of a scan. The reference's only knob here is
+ * the chunk size of its launches (-i, src/EpiOpenCL.c:1559-1607); these
+ * settings replace it. Tiles are 64 b-SNPs wide; a FAST tile (no missing
+ * call in either SNP range) is fast_rows a-SNPs high, a GENERAL tile 16. */
+typedef struct epi_schedule {
+    int32_t fast_rows;  /* MFMA form: 64 or 32; plain form: 32 or 16 */
+    int32_t super_a;    /* tiles are issued in super-tiles of super_a x super_b */
+    int32_t super_b;    /*   tiles (1..64 each) for L2 reuse */
+    int32_t queues;     /* batch only: 8 = one work queue per XCD, 1 = one shared queue */
+    int32_t order;      /* batch only: 0 locality, 1 GENERAL tiles first,
+                           2 datasets by decreasing nid */
+    int32_t reserved[3];
+} epi_schedule;
+
+/* Schedule of single scans of one form (default: fast_rows 64 for MFMA, 32
+ * for plain, super-tiles 8 x 4). */
+int epi_set_schedule(epi_ctx *ctx, int32_t form, const epi_schedule *s);
+int epi_get_schedule(epi_ctx *ctx, int32_t form, epi_schedule *s);
+
 /* Device time (ms, HIP events on the context's stream) of the pair kernels
  * of the last epi_scan, and the number of pairs it tested. */
 int epi_last_scan_stats(epi_ctx *ctx, double *kernel_ms, int64_t *pairs, int32_t *launches);
@@ -121,6 +140,85 @@ int epi_last_scan_stats(epi_ctx *ctx, double *kernel_ms, int64_t *pairs, int32_t
 int epi_pair_dump(epi_ctx *ctx, const epi_scan_params *p, int32_t a0, int32_t a1,
                   int32_t b0, int32_t b1, int32_t *counts, double *sums, double *F,
                   double *F2);
+
+/* ---------------------------------------------------------------- batch ---
+ * Many scans in one persistent launch: datasets of different nid and nsnp,
+ * each with one or more phenotypes (a scan is one (dataset, phenotype)
+ * pair). Replaces a loop of perform_scan calls (src/EpiOpenCL.c:1312-1686),
+ * one per dataset, per phenotype file (-P, src/main.c:172-181, :295-298) and
+ * per permuted phenotype (-p, src/main.c:101-112, :309-312), with a single
+ * device launch: the work items of every scan are dealt to work queues (by
+ * default eight, one per XCD, of equal cost) and pulled by a grid of
+ * occupancy x CU workgroups. The first epi_batch_scan with given test,
+ * form and thresholds captures the launch (queue reset + scan kernel) as a
+ * HIP graph; later calls with the same settings replay it.
+ * A batch shares its context's stream and device; destroy it before the
+ * context. */
+typedef struct epi_batch epi_batch;
+
+int epi_batch_create(epi_ctx *ctx, epi_batch **batch);
+void epi_batch_destroy(epi_batch *batch);
+
+/* Add a dataset: genotypes in layout A1 (as epi_set_genotypes) and its
+ * chromosome blocks (as epi_set_chromosomes; chrstart == NULL means one block
+ * over all SNPs). *dataset receives its index. */
+int epi_batch_add_dataset(epi_batch *batch, const int32_t *words, int32_t nid, int32_t nsnp,
+                          const int32_t *chrstart, const int32_t *chrsize, int32_t nchr,
+                          int32_t *dataset);
+
+/* Add nphen phenotype vectors of the dataset's nid (as epi_set_phenotypes);
+ * each becomes one scan, numbered from *first_scan. Once per dataset. */
+int epi_batch_add_phenotypes(epi_batch *batch, int32_t dataset, const float *phen,
+                             int32_t nphen, int32_t *first_scan);
+
+/* Device capacity of the batch's shared hit buffer (records); default 1<<22. */
+int epi_batch_set_hit_capacity(epi_batch *batch, int64_t cap);
+
+/* Scan every pair of every block of every scan with the test, form and
+ * thresholds of p (p->phen and the block range are ignored). */
+int epi_batch_scan(epi_batch *batch, const epi_scan_params *p);
+
+/* Hits of one scan (scan = -1: all scans together) of the last
+ * epi_batch_scan, and whether the shared buffer overflowed. */
+int epi_batch_hit_count(epi_batch *batch, int32_t scan, int64_t *nhits, int32_t *overflow);
+
+/* As epi_get_hits, for one scan of the last epi_batch_scan. */
+int epi_batch_get_hits(epi_batch *batch, int32_t scan, epi_hit *hits, double *F64,
+                       double *F2_64, int32_t *block, int64_t cap, int64_t *n);
+
+/* Device time of the last epi_batch_scan (ms, HIP events around the graph
+ * launch), pairs tested over all scans, work items, graph captures made so
+ * far and the persistent grid (workgroups) of the last capture. */
+int epi_batch_stats(epi_batch *batch, double *kernel_ms, int64_t *pairs, int64_t *tiles,
+                    int32_t *captures, int32_t *grid);
+
+/* Schedule of the batch's scans of one form (defaults as epi_set_schedule,
+ * 8 queues, locality order). */
+int epi_batch_set_schedule(epi_batch *batch, int32_t form, const epi_schedule *s);
+int epi_batch_get_schedule(epi_batch *batch, int32_t form, epi_schedule *s);
+
+/* Autotuner dimensions. */
+#define EPI_TUNE_ROWS 1    /* fast_rows */
+#define EPI_TUNE_SUPER 2   /* super_a x super_b in {8x4, 4x4, 16x2, 4x8, 1x1} */
+#define EPI_TUNE_QUEUES 4  /* queues in {8, 1} */
+#define EPI_TUNE_ORDER 8   /* order in {0, 1, 2} */
+#define EPI_TUNE_ALL 15
+
+typedef struct epi_tune_entry {
+    epi_schedule sched;
+    double ms;          /* median device time of the whole batch */
+} epi_tune_entry;
+
+/* Pick the batch's schedule for p->form by measurement: starting from the
+ * current schedule, each dimension in `dims` (0 = all) is varied in turn
+ * and the fastest value kept (coordinate descent). Every candidate runs
+ * the whole batch as a captured graph: one warm-up replay, then the median
+ * of `reps` timed replays. The winner becomes the batch's schedule for
+ * p->form. `log` (optional) receives up to log_cap measured candidates,
+ * *nlog the number measured. Hit buffers hold the last candidate's hits. */
+int epi_batch_autotune(epi_batch *batch, const epi_scan_params *p, int32_t dims, int32_t reps,
+                       epi_schedule *best, double *best_ms, epi_tune_entry *log,
+                       int32_t log_cap, int32_t *nlog);
 
 #ifdef __cplusplus
 }
