@@ -76,6 +76,69 @@ def accuracy(s, wl, form):
             "f2_relerr_max": float(rel(F2.ravel(), ref["F2"]).max())}
 
 
+def run_batch(E, s, steps, warmup, tune=True, plain=True, check=True):
+    """c4: 64 independent scans of uneven N and M in one captured persistent launch per step."""
+    from epigpu_amd import workloads
+    bw = workloads.get_batch("c4")
+    t0 = time.perf_counter()
+    bw.generate()
+    log(f"[bench c4] generated {len(bw.scans)} scans, {bw.pairs} pairs in {time.perf_counter() - t0:.1f} s")
+    b = E.Batch(s)
+    t0 = time.perf_counter()
+    for w in bw.scans:
+        d = b.add_dataset(w.words, w.nid, w.chr_starts, w.chr_sizes)
+        b.add_phenotypes(d, w.y)
+    prep_s = time.perf_counter() - t0
+    t1, t2 = bw.scans[0].t1, bw.scans[0].t2
+    out = {"workload": "c4: 64 scans, N~U[300,6000], M~U[1000,30000] over 1-4 chr, N(0,1) 3% NaN, test i, "
+                       f"t1={t1:g}, t2={t2:g}", "scans": len(bw.scans), "pairs_per_step": bw.pairs,
+           "prep_ms": prep_s * 1e3}
+    first = b.scan(test="i", form="mfma", t1=t1, t2=t2)
+    first = b.scan(test="i", form="mfma", t1=t1, t2=t2)
+    out["default_schedule"] = dict(b.schedule("mfma"), kernel_ms=first["kernel_ms"])
+    log(f"[bench c4] default schedule {out['default_schedule']}")
+    if tune:
+        t0 = time.perf_counter()
+        res = b.autotune(test="i", form="mfma", t1=t1, t2=t2, reps=1)
+        out["autotune"] = {"best": res["best"], "best_ms": res["best_ms"], "candidates": len(res["log"]),
+                           "tune_s": time.perf_counter() - t0, "log": res["log"]}
+        log(f"[bench c4] autotune: best {res['best']} {res['best_ms']:.1f} ms over {len(res['log'])} candidates")
+    for i in range(warmup):
+        b.scan(test="i", form="mfma", t1=t1, t2=t2)
+    kms = []
+    t0 = time.perf_counter()
+    for i in range(steps):
+        st = b.scan(test="i", form="mfma", t1=t1, t2=t2)
+        kms.append(st["kernel_ms"])
+        log(f"[bench c4] step {i}: {st['kernel_ms']:.1f} ms kernel, {st['nhits']} hits")
+    elapsed = time.perf_counter() - t0
+    out.update(value=bw.pairs * steps / elapsed, unit="pair-tests/s", ms_per_step=elapsed * 1e3 / steps,
+               kernel_ms_per_step=float(np.mean(kms)), schedule=b.schedule("mfma"), grid=st["grid"],
+               captures=st["captures"], hits=st["nhits"])
+    if check:
+        # two scans of the batch against single scans of the same data
+        same = True
+        for i in (0, len(bw.scans) // 2):
+            w = bw.scans[i]
+            g = b.hits(i)
+            s.set_genotypes(w.words, w.nid)
+            s.set_phenotypes(w.y)
+            s.set_chromosomes(w.chr_starts, w.chr_sizes)
+            one = s.scan(test="i", form="mfma", t1=t1, t2=t2)
+            same &= bool((g["hits"]["snp1"] == one["hits"]["snp1"]).all() and
+                         (g["hits"]["snp2"] == one["hits"]["snp2"]).all() and len(g["hits"]) == len(one["hits"]))
+        out["same_hits_as_single_scans"] = same
+    if plain:
+        r2 = b.scan(test="i", form="plain", t1=t1, t2=t2)
+        r2 = b.scan(test="i", form="plain", t1=t1, t2=t2)
+        out["plain_kernel_ms"] = r2["kernel_ms"]
+        out["mfma_over_plain"] = r2["kernel_ms"] / out["kernel_ms_per_step"]
+        out["plain_same_hit_count"] = r2["nhits"] == out["hits"]
+        log(f"[bench c4] plain {r2['kernel_ms']:.1f} ms; mfma/plain {out['mfma_over_plain']:.2f}x")
+    b.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +150,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-check", action="store_true", help="skip the oracle check")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched c4 measurement")
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--fast-rows", type=int, default=0, help="FAST tile height of the single scan (0: default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run an elementwise copy of MIB MiB (FP32 y = x + 1) for HBM counter calibration")
     args = ap.parse_args()
@@ -117,6 +183,8 @@ def main():
 
     s = E.Scanner(0)
     log(f"[bench] device {s.device_name()}")
+    if args.fast_rows:
+        s.set_schedule(args.form, fast_rows=args.fast_rows)
     t0 = time.perf_counter()
     s.set_genotypes(wl.words, wl.nid)
     s.set_phenotypes(wl.y)
@@ -175,6 +243,9 @@ def main():
     if not args.no_check:
         out["check"] = accuracy(s, wl, args.form)
         log(f"[bench] oracle check: {out['check']}")
+    out["config"]["schedule"] = s.schedule(args.form)
+    if not args.no_batch:
+        out["batch_c4"] = run_batch(E, s, args.batch_steps, 1, plain=not args.no_plain, check=not args.no_check)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", min(16, os.cpu_count() or 1)))
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds, threads)

@@ -4,6 +4,8 @@ c1  N=500,   M=2000 (1 chr),        MAF U(0.05,0.5), N(0,1),                    
 c2  N=4000,  M=20000 (2 x 10000),   MAF U(0.01,0.5), 1% missing calls, N(0,1) 2% NaN, 8df+4df
 c3  N=8000,  M=50000 (1 chr),       MAF U(0.05,0.5), (Gamma(2,1)-2)/sqrt2 + 10 planted
                                     pairs (+0.15 SD in one 3x3 cell),             8df+4df
+c4  64 independent scans, N ~ U[300,6000], M ~ U[1000,30000] over 1-4 chromosomes,
+    MAF U(0.05,0.5), N(0,1) with 3% NaN,                                      8df+4df
 c5  N=20000, M=200000 (1 chr),      MAF U(0.01,0.5), 1% missing, 4 phenotypes     8df+4df
 Seeds are 0xE9150000 + config number. Thresholds default to the reference's
 6.5 / 10.5 (src/main.c:248-249) unless the workload fixes a p-value.
@@ -75,3 +77,38 @@ def get(name):
                         [(S.PHEN_NORMAL, 0.0), (S.PHEN_GAMMA2, 0.0), (S.PHEN_T5, 0.0), (S.PHEN_BERN02, 0.0)],
                         "i", 6.5, 10.5, seed=seed)
     raise KeyError(name)
+
+
+@dataclass
+class BatchWorkload:
+    """Many independent scans (c4): scan i has its own genotypes, chromosomes and phenotype."""
+    name: str
+    scans: List[Workload]
+
+    @property
+    def pairs(self):
+        return int(sum(w.pairs for w in self.scans))
+
+    def generate(self):
+        for w in self.scans:
+            w.generate()
+        return self
+
+
+def get_batch(name="c4", nscan=64):
+    """c4: nscan scans with N ~ U[300,6000], M ~ U[1000,30000] split over 1-4 chromosomes (so both
+    intra- and inter-chromosome blocks occur), no missing calls, N(0,1) phenotypes with 3% NaN."""
+    if name != "c4":
+        raise KeyError(name)
+    seed = SEED0 + 4
+    rng = np.random.default_rng(seed)
+    scans = []
+    for i in range(nscan):
+        nid = int(rng.integers(300, 6001))
+        nsnp = int(rng.integers(1000, 30001))
+        nchr = int(rng.integers(1, 5))
+        cuts = np.sort(rng.choice(np.arange(1, nsnp), nchr - 1, replace=False)) if nchr > 1 else np.array([], int)
+        sizes = np.diff(np.concatenate([[0], cuts, [nsnp]])).astype(int).tolist()
+        scans.append(Workload(f"c4.{i}", nid, sizes, (0.05, 0.5), 0.0, [(S.PHEN_NORMAL, 0.03)], "i", 6.5, 10.5,
+                              seed=seed * 1000 + i))
+    return BatchWorkload(name, scans)
