@@ -60,15 +60,15 @@ def cpu_baseline(wl, seconds, threads):
             "sample": f"{len(tl)} seeded 128x128 off-diagonal tiles of the workload ({npairs} pairs, {dt:.1f} s)"}
 
 
-def accuracy(s, wl, form):
+def accuracy(s, wl, form, phen=0):
     """One 64x64 block of pairs against the FP64 oracle."""
     import oracle as O
     a0, b0 = wl.nsnp // 2 + 17, wl.nsnp // 3 + 5
-    counts, sums, F, F2 = s.pair_dump(a0, a0 + 64, b0, b0 + 64, test=wl.test, form=form)
+    counts, sums, F, F2 = s.pair_dump(a0, a0 + 64, b0, b0 + 64, test=wl.test, form=form, phen=phen)
     codes = O.unpack(wl.words[np.r_[a0:a0 + 64, b0:b0 + 64]], wl.nid)
     aa, bb = np.meshgrid(np.arange(64), np.arange(64, 128), indexing="ij")
     pr = np.stack([aa.ravel(), bb.ravel()], 1)
-    ref = O.pairs(codes, wl.y[0], pr)
+    ref = O.pairs(codes, wl.y[phen], pr)
     c = counts.reshape(-1, 9)
     rel = lambda x, r: np.abs(x - r) / np.maximum(np.abs(r), 1.0)
     return {"pairs_checked": int(len(pr)), "counts_exact": bool((c == ref["counts"]).all()),
@@ -139,6 +139,56 @@ def run_batch(E, s, steps, warmup, tune=True, plain=True, check=True):
     return out
 
 
+def run_multi(E, s, steps, warmup, nsnp=0, check=True, sched=None):
+    """c5: one dataset (N=20000, M=200000, 1% missing calls) with 4 phenotypes (N(0,1), Gamma(2,1), t(5),
+    0/1 at prevalence 0.2) as 4 scans of one captured persistent launch sharing the genotype planes.
+    nsnp > 0 scans only the first nsnp SNPs (a bounded sample of the same workload)."""
+    from epigpu_amd import workloads
+    wl = workloads.get("c5")
+    if nsnp:
+        wl.chr_sizes = [nsnp]
+    t0 = time.perf_counter()
+    wl.generate()
+    log(f"[bench c5] generated N={wl.nid} M={wl.nsnp} x {len(wl.y)} phenotypes in {time.perf_counter() - t0:.1f} s")
+    b = E.Batch(s)
+    t0 = time.perf_counter()
+    d = b.add_dataset(wl.words, wl.nid, wl.chr_starts, wl.chr_sizes)
+    scans = b.add_phenotypes(d, wl.y)
+    prep_s = time.perf_counter() - t0
+    if sched:
+        b.set_schedule("mfma", **sched)
+    tests = wl.pairs * len(scans)
+    for i in range(max(1, warmup)):
+        st = b.scan(test=wl.test, form="mfma", t1=wl.t1, t2=wl.t2)
+        log(f"[bench c5] warmup {i}: {st['kernel_ms']:.1f} ms kernel, {st['nhits']} hits, {st['tiles']} items")
+    kms = []
+    t0 = time.perf_counter()
+    for i in range(steps):
+        st = b.scan(test=wl.test, form="mfma", t1=wl.t1, t2=wl.t2)
+        kms.append(st["kernel_ms"])
+        log(f"[bench c5] step {i}: {st['kernel_ms']:.1f} ms kernel, {st['nhits']} hits")
+    elapsed = time.perf_counter() - t0
+    kernel_ms = float(np.mean(kms))
+    out = {"workload": f"c5: N={wl.nid}, M={wl.nsnp} (1 chr), {wl.pairs} pairs x {len(scans)} phenotypes "
+                       f"(N(0,1), Gamma(2,1), t(5), Bernoulli(0.2)), genotype missing {wl.gmiss}, test {wl.test}",
+           "pair_tests_per_step": tests, "value": tests * steps / elapsed, "unit": "pair-tests/s",
+           "ms_per_step": elapsed * 1e3 / steps, "kernel_ms_per_step": kernel_ms, "prep_ms": prep_s * 1e3,
+           "items": st["tiles"], "grid": st["grid"], "schedule": b.schedule("mfma"),
+           "hits_per_phenotype": [b.hit_count(sc)[0] for sc in scans]}
+    logical = 36.0 * wl.nid * tests
+    out["roofline"] = {"bound": "mfma", "achieved": logical / (kernel_ms * 1e-3) / 1e12, "peak": INT8_PEAK_TOPS,
+                       "unit": "TFLOP/s"}
+    out["roofline"]["frac"] = out["roofline"]["achieved"] / INT8_PEAK_TOPS
+    if check:
+        s.set_genotypes(wl.words, wl.nid)
+        s.set_phenotypes(wl.y)
+        s.set_chromosomes(wl.chr_starts, wl.chr_sizes)
+        out["check"] = [accuracy(s, wl, "mfma", r) for r in range(len(scans))]
+        log(f"[bench c5] oracle check {out['check']}")
+    b.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,8 +201,10 @@ def main():
     ap.add_argument("--no-check", action="store_true", help="skip the oracle check")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched c4 measurement")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default schedule (no autotuning)")
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--fast-rows", type=int, default=0, help="FAST tile height of the single scan (0: default)")
+    ap.add_argument("--c5-snps", type=int, default=0, help="c5: scan only the first K SNPs (0: all 200000)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run an elementwise copy of MIB MiB (FP32 y = x + 1) for HBM counter calibration")
     args = ap.parse_args()
@@ -176,6 +228,21 @@ def main():
     import epigpu_amd as E
     from epigpu_amd import workloads
 
+    if args.config in ("c4", "c5"):
+        s = E.Scanner(0)
+        log(f"[bench] device {s.device_name()}")
+        if args.config == "c4":
+            r = run_batch(E, s, args.steps, args.warmup, plain=not args.no_plain, check=not args.no_check)
+        else:
+            r = run_multi(E, s, args.steps, args.warmup, nsnp=args.c5_snps, check=not args.no_check)
+        s.close()
+        out = {"metric": METRIC, "value": r["value"], "unit": "pair-tests/s", "n_gpus": 1, "steps": args.steps,
+               "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "int8", "data": "synthetic (seeded generator, see epigpu_amd/workloads.py)",
+               "config": {"workload": r.pop("workload"), "form": "mfma"}, "detail": r}
+        print(json.dumps(out), flush=True)
+        return
+
     wl = workloads.get(args.config)
     t0 = time.perf_counter()
     wl.generate()
@@ -194,6 +261,14 @@ def main():
 
     def step(form):
         return s.scan(test=wl.test, form=form, t1=wl.t1, t2=wl.t2)
+
+    tuned = None
+    if not args.no_tune and not args.fast_rows:
+        t0 = time.perf_counter()
+        tuned = s.autotune(test=wl.test, form=args.form, t1=wl.t1, t2=wl.t2, reps=1)
+        tuned["tune_s"] = time.perf_counter() - t0
+        log(f"[bench] autotune: {tuned['best']} {tuned['best_ms']:.1f} ms over {len(tuned['log'])} candidates "
+            f"in {tuned['tune_s']:.1f} s")
 
     for i in range(args.warmup):
         r = step(args.form)
@@ -244,6 +319,8 @@ def main():
         out["check"] = accuracy(s, wl, args.form)
         log(f"[bench] oracle check: {out['check']}")
     out["config"]["schedule"] = s.schedule(args.form)
+    if tuned:
+        out["autotune"] = tuned
     if not args.no_batch:
         out["batch_c4"] = run_batch(E, s, args.batch_steps, 1, plain=not args.no_plain, check=not args.no_check)
     if not args.no_cpu:

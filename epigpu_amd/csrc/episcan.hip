@@ -416,7 +416,9 @@ epi_schedule default_schedule(int form)
 {
     epi_schedule s;
     memset(&s, 0, sizeof(s));
-    s.fast_rows = form == EPI_FORM_MFMA ? 64 : 32;
+    // 32-row MFMA tiles keep two workgroups per CU (measured faster than 64
+    // rows at N=4000 with missing calls and at N=8000 without)
+    s.fast_rows = 32;
     s.super_a = 8;
     s.super_b = 4;
     s.queues = 8;
@@ -857,6 +859,101 @@ int batch_fetch(epi_batch *b)
     return EPI_OK;
 }
 
+// One scan of the block range of p with schedule sc: tile list, upload,
+// pair kernels, hit count. Records the device time and counts in c.
+int run_scan(epi_ctx *c, const epi_scan_params *p, const epi_schedule &sc)
+{
+    int rc;
+    int nchr = (int)c->d.chrstart.size();
+    int nblk = nchr * (nchr + 1) / 2;
+    int bb = std::max(0, p->blk_begin), be = p->blk_end <= 0 ? nblk : std::min(nblk, p->blk_end);
+    TileLists tl;
+    build_tiles(c->d, bb, be, p->form, sc, tl);
+    if ((rc = upload_tiles(c, tl))) return rc;
+    HIPCHK(c, hipMemsetAsync(c->nhits, 0, sizeof(unsigned long long), c->stream));
+    DumpSink ds;
+    memset(&ds, 0, sizeof(ds));
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    if ((rc = launch_pairs<false>(c, p, sc, tl, ds))) return rc;
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    unsigned long long nh = 0;
+    HIPCHK(c, hipMemcpyAsync(&nh, c->nhits, sizeof(nh), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    c->last_pairs = tl.pairs;
+    c->last_nhits = (long long)nh;
+    return EPI_OK;
+}
+
+// Coordinate descent over the schedule dimensions in `dims`: each dimension
+// in turn is set to every candidate value (others fixed at the best so far)
+// and the fastest is kept. measure(s, ms) times one schedule; repeated
+// schedules are timed once. Candidates that measure equal keep the earlier.
+template <class Measure>
+int tune_descent(int form, int dims, epi_schedule &cur, double &tcur, int &nmeasured, epi_tune_entry *log,
+                 int log_cap, Measure measure)
+{
+    std::vector<std::pair<epi_schedule, double>> seen;
+    auto timed = [&](const epi_schedule &s, double &ms) -> int {
+        for (auto &e : seen)
+            if (same_schedule(e.first, s)) {
+                ms = e.second;
+                return EPI_OK;
+            }
+        int r = measure(s, ms);
+        if (r) return r;
+        seen.push_back({s, ms});
+        if (log && nmeasured < log_cap) {
+            log[nmeasured].sched = s;
+            log[nmeasured].ms = ms;
+        }
+        nmeasured++;
+        return EPI_OK;
+    };
+    int rc = timed(cur, tcur);
+    if (rc) return rc;
+    auto tryit = [&](const epi_schedule &s) -> int {
+        double t = 0.0;
+        int r = timed(s, t);
+        if (r) return r;
+        if (t < tcur) {
+            cur = s;
+            tcur = t;
+        }
+        return EPI_OK;
+    };
+    const int rows_m[] = {32, 64}, rows_p[] = {32, 16};
+    const int sup[][2] = {{8, 4}, {4, 4}, {16, 2}, {4, 8}, {1, 1}};
+    if (dims & EPI_TUNE_ROWS)
+        for (int v : (form == EPI_FORM_MFMA ? rows_m : rows_p)) {
+            epi_schedule s = cur;
+            s.fast_rows = v;
+            if ((rc = tryit(s))) return rc;
+        }
+    if (dims & EPI_TUNE_SUPER)
+        for (auto &v : sup) {
+            epi_schedule s = cur;
+            s.super_a = v[0];
+            s.super_b = v[1];
+            if ((rc = tryit(s))) return rc;
+        }
+    if (dims & EPI_TUNE_QUEUES)
+        for (int v : {8, 1}) {
+            epi_schedule s = cur;
+            s.queues = v;
+            if ((rc = tryit(s))) return rc;
+        }
+    if (dims & EPI_TUNE_ORDER)
+        for (int v : {0, 1, 2}) {
+            epi_schedule s = cur;
+            s.order = v;
+            if ((rc = tryit(s))) return rc;
+        }
+    return EPI_OK;
+}
+
 }  // namespace
 
 // ============================================================== C ABI ====
@@ -983,27 +1080,40 @@ int epi_scan(epi_ctx *c, const epi_scan_params *p)
     if (rc) return rc;
     HIPCHK(c, hipSetDevice(c->device));
     if ((rc = ds_marginals(c, c->d))) return rc;
-    int nchr = (int)c->d.chrstart.size();
-    int nblk = nchr * (nchr + 1) / 2;
-    int bb = std::max(0, p->blk_begin), be = p->blk_end <= 0 ? nblk : std::min(nblk, p->blk_end);
-    TileLists tl;
-    const epi_schedule &sc = c->sched[p->form];
-    build_tiles(c->d, bb, be, p->form, sc, tl);
-    if ((rc = upload_tiles(c, tl))) return rc;
-    HIPCHK(c, hipMemsetAsync(c->nhits, 0, sizeof(unsigned long long), c->stream));
-    DumpSink ds;
-    memset(&ds, 0, sizeof(ds));
-    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    if ((rc = launch_pairs<false>(c, p, sc, tl, ds))) return rc;
-    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-    unsigned long long nh = 0;
-    HIPCHK(c, hipMemcpyAsync(&nh, c->nhits, sizeof(nh), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    float ms = 0.0f;
-    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    c->last_ms = ms;
-    c->last_pairs = tl.pairs;
-    c->last_nhits = (long long)nh;
+    return run_scan(c, p, c->sched[p->form]);
+}
+
+int epi_autotune(epi_ctx *c, const epi_scan_params *p, int32_t dims, int32_t reps, epi_schedule *best,
+                 double *best_ms, epi_tune_entry *log, int32_t log_cap, int32_t *nlog)
+{
+    if (!c) return EPI_ERR_ARG;
+    int rc = check_params(c, p);
+    if (rc) return rc;
+    if (reps < 1 || reps > 100) return fail(c, EPI_ERR_ARG, "reps must be in [1, 100]");
+    if (dims == 0) dims = EPI_TUNE_ROWS | EPI_TUNE_SUPER;
+    dims &= EPI_TUNE_ROWS | EPI_TUNE_SUPER;  // queues and order do not apply to single scans
+    HIPCHK(c, hipSetDevice(c->device));
+    if ((rc = ds_marginals(c, c->d))) return rc;
+    epi_schedule cur = c->sched[p->form];
+    double tcur = 0.0;
+    int n = 0;
+    rc = tune_descent(p->form, dims, cur, tcur, n, log, log_cap, [&](const epi_schedule &s, double &ms) -> int {
+        int r = run_scan(c, p, s);  // warm-up
+        if (r) return r;
+        std::vector<double> t(reps);
+        for (int i = 0; i < reps; i++) {
+            if ((r = run_scan(c, p, s))) return r;
+            t[i] = c->last_ms;
+        }
+        std::sort(t.begin(), t.end());
+        ms = t[reps / 2];
+        return EPI_OK;
+    });
+    if (rc) return rc;
+    c->sched[p->form] = cur;
+    if (best) *best = cur;
+    if (best_ms) *best_ms = tcur;
+    if (nlog) *nlog = n;
     return EPI_OK;
 }
 
@@ -1256,75 +1366,22 @@ int epi_batch_autotune(epi_batch *b, const epi_scan_params *p, int32_t dims, int
     if (dims == 0) dims = EPI_TUNE_ALL;
     HIPCHK(c, hipSetDevice(c->device));
     const int form = p->form;
+    epi_schedule cur = b->sched[form];
+    double tcur = 0.0;
     int n = 0;
-    std::vector<std::pair<epi_schedule, double>> seen;
-    // median device time of reps replays after one warm-up replay
-    auto measure = [&](const epi_schedule &s, double &ms) -> int {
-        for (auto &e : seen)
-            if (same_schedule(e.first, s)) {
-                ms = e.second;
-                return EPI_OK;
-            }
+    rc = tune_descent(form, dims, cur, tcur, n, log, log_cap, [&](const epi_schedule &s, double &ms) -> int {
         b->sched[form] = s;
         int r = batch_prepare(b, p);
         if (r) return r;
-        if ((r = batch_run(b, nullptr))) return r;
+        if ((r = batch_run(b, nullptr))) return r;  // warm-up
         std::vector<double> t(reps);
         for (int i = 0; i < reps; i++)
             if ((r = batch_run(b, &t[i]))) return r;
         std::sort(t.begin(), t.end());
         ms = t[reps / 2];
-        seen.push_back({s, ms});
-        if (log && n < log_cap) {
-            log[n].sched = s;
-            log[n].ms = ms;
-        }
-        n++;
         return EPI_OK;
-    };
-    epi_schedule cur = b->sched[form];
-    double tcur = 0.0;
-    if ((rc = measure(cur, tcur))) return rc;
-    // coordinate descent: each dimension in turn, keeping the best value
-    const int rows_m[] = {64, 32}, rows_p[] = {32, 16};
-    const int sup[][2] = {{8, 4}, {4, 4}, {16, 2}, {4, 8}, {1, 1}};
-    const int queues[] = {8, 1};
-    const int orders[] = {0, 1, 2};
-    auto tryit = [&](const epi_schedule &s) -> int {
-        double t = 0.0;
-        int r = measure(s, t);
-        if (r) return r;
-        if (t < tcur) {
-            cur = s;
-            tcur = t;
-        }
-        return EPI_OK;
-    };
-    if (dims & EPI_TUNE_ROWS)
-        for (int v : (form == EPI_FORM_MFMA ? rows_m : rows_p)) {
-            epi_schedule s = cur;
-            s.fast_rows = v;
-            if ((rc = tryit(s))) return rc;
-        }
-    if (dims & EPI_TUNE_SUPER)
-        for (auto &v : sup) {
-            epi_schedule s = cur;
-            s.super_a = v[0];
-            s.super_b = v[1];
-            if ((rc = tryit(s))) return rc;
-        }
-    if (dims & EPI_TUNE_QUEUES)
-        for (int v : queues) {
-            epi_schedule s = cur;
-            s.queues = v;
-            if ((rc = tryit(s))) return rc;
-        }
-    if (dims & EPI_TUNE_ORDER)
-        for (int v : orders) {
-            epi_schedule s = cur;
-            s.order = v;
-            if ((rc = tryit(s))) return rc;
-        }
+    });
+    if (rc) return rc;
     b->sched[form] = cur;
     if (best) *best = cur;
     if (best_ms) *best_ms = tcur;

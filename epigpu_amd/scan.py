@@ -136,6 +136,8 @@ def _declare(name, L):
             "epi_batch_get_hits": (ctypes.c_int, [vp, i32, vp, vp, vp, vp, i64, _p(i64)]),
             "epi_batch_stats": (ctypes.c_int, [vp, _p(f64), _p(i64), _p(i64), _p(i32), _p(i32)]),
             "epi_set_schedule": (ctypes.c_int, [vp, i32, _p(Schedule)]),
+            "epi_autotune": (ctypes.c_int, [vp, _p(ScanParams), i32, i32, _p(Schedule), _p(f64), _p(TuneEntry), i32,
+                                            _p(i32)]),
             "epi_get_schedule": (ctypes.c_int, [vp, i32, _p(Schedule)]),
             "epi_batch_set_schedule": (ctypes.c_int, [vp, i32, _p(Schedule)]),
             "epi_batch_get_schedule": (ctypes.c_int, [vp, i32, _p(Schedule)]),
@@ -407,6 +409,16 @@ class Scanner:
         for k, v in kw.items():
             setattr(sc, k, v)
         self._chk(self.L.epi_set_schedule(self.ctx, FORMS[form], ctypes.byref(sc)), "epi_set_schedule")
+
+    def autotune(self, test="f", form="mfma", t1=6.5, t2=10.5, phen=0, dims=0, reps=1):
+        """Measure candidate single-scan schedules (coordinate descent) and keep the fastest."""
+        p = self.params(test, form, t1, t2, phen)
+        best, ms, n = Schedule(), ctypes.c_double(), ctypes.c_int32()
+        log = (TuneEntry * 64)()
+        self._chk(self.L.epi_autotune(self.ctx, ctypes.byref(p), dims, reps, ctypes.byref(best), ctypes.byref(ms),
+                                      log, 64, ctypes.byref(n)), "epi_autotune")
+        return {"best": best.as_dict(), "best_ms": ms.value,
+                "log": [dict(log[i].sched.as_dict(), ms=log[i].ms) for i in range(min(n.value, 64))]}
 
     @staticmethod
     def params(test="f", form="mfma", t1=6.5, t2=10.5, phen=0, blk_begin=0, blk_end=0):

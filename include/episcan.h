@@ -122,10 +122,34 @@ typedef struct epi_schedule {
     int32_t reserved[3];
 } epi_schedule;
 
-/* Schedule of single scans of one form (default: fast_rows 64 for MFMA, 32
- * for plain, super-tiles 8 x 4). */
+/* Schedule of single scans of one form (default: fast_rows 32, super-tiles
+ * 8 x 4). */
 int epi_set_schedule(epi_ctx *ctx, int32_t form, const epi_schedule *s);
 int epi_get_schedule(epi_ctx *ctx, int32_t form, epi_schedule *s);
+
+/* Autotuner dimensions. */
+#define EPI_TUNE_ROWS 1    /* fast_rows */
+#define EPI_TUNE_SUPER 2   /* super_a x super_b in {8x4, 4x4, 16x2, 4x8, 1x1} */
+#define EPI_TUNE_QUEUES 4  /* queues in {8, 1} */
+#define EPI_TUNE_ORDER 8   /* order in {0, 1, 2} */
+#define EPI_TUNE_ALL 15
+
+typedef struct epi_tune_entry {
+    epi_schedule sched;
+    double ms;          /* median device time of the scan */
+} epi_tune_entry;
+
+/* Pick the single-scan schedule for p->form by measurement: starting from
+ * the current schedule, each dimension in `dims` (EPI_TUNE_ROWS and
+ * EPI_TUNE_SUPER apply; 0 = both) is varied in turn and the fastest value
+ * kept (coordinate descent). Every candidate runs the scan of p once as a
+ * warm-up, then `reps` timed times (median). The winner becomes the
+ * context's schedule for p->form. `log` (optional) receives up to log_cap
+ * measured candidates, *nlog the number measured. Hit buffers hold the last
+ * candidate's hits. */
+int epi_autotune(epi_ctx *ctx, const epi_scan_params *p, int32_t dims, int32_t reps,
+                 epi_schedule *best, double *best_ms, epi_tune_entry *log, int32_t log_cap,
+                 int32_t *nlog);
 
 /* Device time (ms, HIP events on the context's stream) of the pair kernels
  * of the last epi_scan, and the number of pairs it tested. */
@@ -196,18 +220,6 @@ int epi_batch_stats(epi_batch *batch, double *kernel_ms, int64_t *pairs, int64_t
  * 8 queues, locality order). */
 int epi_batch_set_schedule(epi_batch *batch, int32_t form, const epi_schedule *s);
 int epi_batch_get_schedule(epi_batch *batch, int32_t form, epi_schedule *s);
-
-/* Autotuner dimensions. */
-#define EPI_TUNE_ROWS 1    /* fast_rows */
-#define EPI_TUNE_SUPER 2   /* super_a x super_b in {8x4, 4x4, 16x2, 4x8, 1x1} */
-#define EPI_TUNE_QUEUES 4  /* queues in {8, 1} */
-#define EPI_TUNE_ORDER 8   /* order in {0, 1, 2} */
-#define EPI_TUNE_ALL 15
-
-typedef struct epi_tune_entry {
-    epi_schedule sched;
-    double ms;          /* median device time of the whole batch */
-} epi_tune_entry;
 
 /* Pick the batch's schedule for p->form by measurement: starting from the
  * current schedule, each dimension in `dims` (0 = all) is varied in turn

@@ -144,7 +144,7 @@ def test_batch_overflow_and_errors(scanner, batch_data):
 
 
 SCHEDULES = {
-    "mfma": [dict(fast_rows=32), dict(fast_rows=64, super_a=1, super_b=1), dict(fast_rows=32, super_a=16, super_b=2)],
+    "mfma": [dict(fast_rows=64), dict(fast_rows=64, super_a=1, super_b=1), dict(fast_rows=32, super_a=16, super_b=2)],
     "plain": [dict(fast_rows=16), dict(fast_rows=32, super_a=4, super_b=8)],
 }
 
@@ -192,3 +192,23 @@ def test_batch_autotune(scanner, batch_data):
         st = b.scan(test="i", form="mfma", t1=T1, t2=T2)
         assert st["nhits"] == base["nhits"] and st["pairs"] == base["pairs"]
     b.close()
+
+
+def test_single_scan_autotune(scanner, batch_data):
+    """The single-scan autotuner keeps the fastest measured schedule and the hits do not change."""
+    nid, words, phen, starts, sizes = batch_data[2]
+    scanner.set_genotypes(words, nid)
+    scanner.set_phenotypes(phen)
+    scanner.set_chromosomes(starts, sizes)
+    default = scanner.schedule("mfma")
+    ref = scanner.scan(test="i", form="mfma", t1=T1, t2=T2)
+    try:
+        res = scanner.autotune(test="i", form="mfma", t1=T1, t2=T2, reps=2)
+        assert len(res["log"]) >= 5
+        assert res["best_ms"] == min(e["ms"] for e in res["log"])
+        assert scanner.schedule("mfma") == res["best"]
+        g = scanner.scan(test="i", form="mfma", t1=T1, t2=T2)
+        assert (g["hits"]["snp1"] == ref["hits"]["snp1"]).all() and (g["hits"]["snp2"] == ref["hits"]["snp2"]).all()
+        assert np.abs(g["F64"] - ref["F64"]).max(initial=0) <= 1e-12 * max(1.0, np.abs(ref["F64"]).max(initial=0))
+    finally:
+        scanner.set_schedule("mfma", **default)
