@@ -76,7 +76,7 @@ def accuracy(s, wl, form, phen=0):
             "f2_relerr_max": float(rel(F2.ravel(), ref["F2"]).max())}
 
 
-def run_batch(E, s, steps, warmup, tune=True, plain=True, check=True):
+def run_batch(E, s, steps, warmup, tune=True, plain=True, check=True, sched=None):
     """c4: 64 independent scans of uneven N and M in one captured persistent launch per step."""
     from epigpu_amd import workloads
     bw = workloads.get_batch("c4")
@@ -93,6 +93,8 @@ def run_batch(E, s, steps, warmup, tune=True, plain=True, check=True):
     out = {"workload": "c4: 64 scans, N~U[300,6000], M~U[1000,30000] over 1-4 chr, N(0,1) 3% NaN, test i, "
                        f"t1={t1:g}, t2={t2:g}", "scans": len(bw.scans), "pairs_per_step": bw.pairs,
            "prep_ms": prep_s * 1e3}
+    if sched:
+        b.set_schedule("mfma", **sched)
     first = b.scan(test="i", form="mfma", t1=t1, t2=t2)
     first = b.scan(test="i", form="mfma", t1=t1, t2=t2)
     out["default_schedule"] = dict(b.schedule("mfma"), kernel_ms=first["kernel_ms"])
@@ -204,6 +206,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default schedule (no autotuning)")
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--fast-rows", type=int, default=0, help="FAST tile height of the single scan (0: default)")
+    ap.add_argument("--schedule", default="", metavar="ROWS,SA,SB[,QUEUES,ORDER]",
+                    help="fix the schedule (FAST tile height, super-tile; batch: queues, order) instead of autotuning")
     ap.add_argument("--c5-snps", type=int, default=0, help="c5: scan only the first K SNPs (0: all 200000)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run an elementwise copy of MIB MiB (FP32 y = x + 1) for HBM counter calibration")
@@ -231,10 +235,15 @@ def main():
     if args.config in ("c4", "c5"):
         s = E.Scanner(0)
         log(f"[bench] device {s.device_name()}")
+        sched = None
+        if args.schedule:
+            v = [int(x) for x in args.schedule.split(",")]
+            sched = dict(zip(("fast_rows", "super_a", "super_b", "queues", "order"), v))
         if args.config == "c4":
-            r = run_batch(E, s, args.steps, args.warmup, plain=not args.no_plain, check=not args.no_check)
+            r = run_batch(E, s, args.steps, args.warmup, tune=not (args.no_tune or sched), plain=not args.no_plain,
+                          check=not args.no_check, sched=sched)
         else:
-            r = run_multi(E, s, args.steps, args.warmup, nsnp=args.c5_snps, check=not args.no_check)
+            r = run_multi(E, s, args.steps, args.warmup, nsnp=args.c5_snps, check=not args.no_check, sched=sched)
         s.close()
         out = {"metric": METRIC, "value": r["value"], "unit": "pair-tests/s", "n_gpus": 1, "steps": args.steps,
                "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
@@ -252,6 +261,9 @@ def main():
     log(f"[bench] device {s.device_name()}")
     if args.fast_rows:
         s.set_schedule(args.form, fast_rows=args.fast_rows)
+    if args.schedule:
+        rows, sa, sb = (int(x) for x in args.schedule.split(",")[:3])
+        s.set_schedule(args.form, fast_rows=rows, super_a=sa, super_b=sb)
     t0 = time.perf_counter()
     s.set_genotypes(wl.words, wl.nid)
     s.set_phenotypes(wl.y)
@@ -263,7 +275,7 @@ def main():
         return s.scan(test=wl.test, form=form, t1=wl.t1, t2=wl.t2)
 
     tuned = None
-    if not args.no_tune and not args.fast_rows:
+    if not args.no_tune and not args.fast_rows and not args.schedule:
         t0 = time.perf_counter()
         tuned = s.autotune(test=wl.test, form=args.form, t1=wl.t1, t2=wl.t2, reps=1)
         tuned["tune_s"] = time.perf_counter() - t0

@@ -2,8 +2,8 @@
 //
 // Device state of one dataset (a context holds one; a batch holds many):
 //   words   [nsnp][npack1] int32   genotypes as uploaded (layout A1)
-//   planes  [nsnp][3][Kp]  uint8   class masks 0xFF/0x00 for classes 0, 1, NM
-//   bits    [3][Kw][nsnp]  uint32  class bitplanes (plain form)
+//   mbits   [3][Kw][nsnp]  uint32  bitplanes of classes 0, 1 and call present (MFMA form)
+//   bits    [3][Kw][nsnp]  uint32  bitplanes of the two rarest classes and call present (plain form)
 //   cls     [nsnp]         int32   plain-form plane -> class map
 //   per phenotype r:
 //     wts   [7][Kp] int8   V, 3 limbs of q, 3 limbs of round(q^2/2^23)
@@ -22,7 +22,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/episcan.h"
+#include "episcan.h"
 #include "kern_batch.h"
 #include "kern_mfma.h"
 #include "kern_plain.h"
@@ -54,7 +54,7 @@ struct PhenDev {
 struct Dataset {
     int nid = 0, nsnp = 0, npack = 0, remain = 0, npack1 = 0, Kp = 0, Kw = 0;
     int32_t *words = nullptr;
-    uint8_t *planes = nullptr;
+    uint32_t *mbits = nullptr;
     uint32_t *bits = nullptr;
     int32_t *dmiss = nullptr;
     int32_t *cls = nullptr;
@@ -85,9 +85,14 @@ struct epi_ctx {
     unsigned long long *nhits = nullptr;
     long long cap = 1ll << 22;
     long long last_nhits = 0;
-    // tiles
+    // tiles; the device list of the last scan is kept and reused while the
+    // dataset, block range, form and schedule stay the same
     int4 *tiles = nullptr;
     size_t tiles_cap = 0;
+    TileLists tl;
+    bool tl_valid = false;
+    int tl_bb = 0, tl_be = 0, tl_form = -1;
+    epi_schedule tl_sched = {};
     // stats
     double last_ms = 0.0;
     long long last_pairs = 0;
@@ -183,7 +188,7 @@ void free_phen(PhenDev &p)
 void free_dataset(Dataset &d)
 {
     dfree(d.words);
-    dfree(d.planes);
+    dfree(d.mbits);
     dfree(d.bits);
     dfree(d.dmiss);
     dfree(d.cls);
@@ -209,21 +214,21 @@ int ds_set_genotypes(epi_ctx *c, Dataset &d, const int32_t *words, int nid, int 
     d.Kw = d.Kp / 32;
     size_t nwords = (size_t)nsnp * d.npack1;
     int rc;
-    if ((rc = dalloc(c, d.words, nwords)) || (rc = dalloc(c, d.planes, (size_t)nsnp * 3 * d.Kp)) ||
+    if ((rc = dalloc(c, d.words, nwords)) || (rc = dalloc(c, d.mbits, (size_t)3 * d.Kw * nsnp)) ||
         (rc = dalloc(c, d.bits, (size_t)3 * d.Kw * nsnp)) || (rc = dalloc(c, d.dmiss, (size_t)nsnp)) ||
         (rc = dalloc(c, d.cls, (size_t)nsnp)))
         return rc;
     HIPCHK(c, hipMemcpyAsync(d.words, words, nwords * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemsetAsync(d.dmiss, 0, (size_t)nsnp * sizeof(int32_t), c->stream));
-    long long total = (long long)(d.Kp / 16) * nsnp;
-    hipLaunchKernelGGL(transcode_planes, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, c->stream,
-                       (const uint32_t *)d.words, nid, d.npack, d.remain, d.npack1, nsnp, d.Kp, d.planes, d.dmiss);
-    HIPCHK(c, hipGetLastError());
     hipLaunchKernelGGL(class_order, dim3((nsnp + 255) / 256), dim3(256), 0, c->stream, (const uint32_t *)d.words, nid,
-                       d.npack, d.remain, d.npack1, nsnp, d.cls);
+                       d.npack, d.remain, d.npack1, nsnp, d.cls, d.dmiss);
     HIPCHK(c, hipGetLastError());
     hipLaunchKernelGGL(transcode_bits, dim3((nsnp + 255) / 256, d.Kw), dim3(256), 0, c->stream,
-                       (const uint32_t *)d.words, nid, d.npack, d.remain, d.npack1, nsnp, d.Kw, d.cls, d.bits);
+                       (const uint32_t *)d.words, nid, d.npack, d.remain, d.npack1, nsnp, d.Kw, (const int32_t *)d.cls,
+                       d.bits);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(transcode_bits, dim3((nsnp + 255) / 256, d.Kw), dim3(256), 0, c->stream,
+                       (const uint32_t *)d.words, nid, d.npack, d.remain, d.npack1, nsnp, d.Kw, (const int32_t *)nullptr,
+                       d.mbits);
     HIPCHK(c, hipGetLastError());
     d.miss.assign(nsnp, 0);
     HIPCHK(c, hipMemcpyAsync(d.miss.data(), d.dmiss, (size_t)nsnp * sizeof(int32_t), hipMemcpyDeviceToHost,
@@ -251,7 +256,7 @@ int ds_set_phenotypes(epi_ctx *c, Dataset &d, const float *phen, int nphen, int 
         d.Kp = Kp;
         d.Kw = Kw;
     }
-    std::vector<int8_t> w((size_t)NW * Kp);
+    std::vector<int8_t> w((size_t)NWP * Kp);  // row NWP - 1 stays zero
     std::vector<int32_t> qv(Kp);
     std::vector<float> yc(Kp);
     std::vector<uint32_t> vb(Kw);
@@ -358,12 +363,13 @@ int ds_set_chromosomes(epi_ctx *c, Dataset &d, const int32_t *chrstart, const in
 int ds_marginals(epi_ctx *c, Dataset &d)
 {
     if (d.marg_ok) return EPI_OK;
-    if (!d.planes || d.ph.empty()) return fail(c, EPI_ERR_STATE, "genotypes and phenotypes must be set");
+    if (!d.mbits || d.ph.empty()) return fail(c, EPI_ERR_STATE, "genotypes and phenotypes must be set");
     for (auto &p : d.ph) {
         int rc = dalloc(c, p.marg, (size_t)d.nsnp);
         if (rc) return rc;
-        hipLaunchKernelGGL(marginals_kernel, dim3(d.nsnp), dim3(256), 0, c->stream, d.planes, d.Kp, d.nsnp, p.wts,
-                           p.qv, p.yc, d.cls, p.marg);
+        hipLaunchKernelGGL(marginals_kernel, dim3((d.nsnp + 63) / 64), dim3(256), 0, c->stream,
+                           (const uint32_t *)d.mbits, d.Kw, d.nsnp, (const uint32_t *)p.vbits, (const int32_t *)p.qv,
+                           (const float *)p.yc, (const int32_t *)d.cls, p.marg);
         HIPCHK(c, hipGetLastError());
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -375,7 +381,7 @@ MfmaScan mfma_scan(const Dataset &d, int r, int tag)
 {
     const PhenDev &p = d.ph[r];
     MfmaScan S;
-    S.planes = d.planes;
+    S.bits = d.mbits;
     S.wts = p.wts;
     S.marg = p.marg;
     S.nsnp = d.nsnp;
@@ -492,6 +498,7 @@ void build_tiles(const Dataset &d, int blk_begin, int blk_end, int form, const e
 
 int upload_tiles(epi_ctx *c, const TileLists &tl)
 {
+    c->tl_valid = false;
     size_t need = tl.fast.size() + tl.gen.size();
     if (need > c->tiles_cap) {
         int rc = dalloc(c, c->tiles, need);
@@ -535,7 +542,7 @@ int check_params(epi_ctx *c, const epi_scan_params *p)
 {
     int rc = check_common(c, p);
     if (rc) return rc;
-    if (!c->d.planes) return fail(c, EPI_ERR_STATE, "genotypes not set");
+    if (!c->d.mbits) return fail(c, EPI_ERR_STATE, "genotypes not set");
     if (p->phen < 0 || p->phen >= (int)c->d.ph.size()) return fail(c, EPI_ERR_ARG, "phenotype index out of range");
     if (c->d.chrstart.empty()) return fail(c, EPI_ERR_STATE, "chromosomes not set");
     return EPI_OK;
@@ -867,9 +874,17 @@ int run_scan(epi_ctx *c, const epi_scan_params *p, const epi_schedule &sc)
     int nchr = (int)c->d.chrstart.size();
     int nblk = nchr * (nchr + 1) / 2;
     int bb = std::max(0, p->blk_begin), be = p->blk_end <= 0 ? nblk : std::min(nblk, p->blk_end);
-    TileLists tl;
-    build_tiles(c->d, bb, be, p->form, sc, tl);
-    if ((rc = upload_tiles(c, tl))) return rc;
+    TileLists &tl = c->tl;
+    if (!(c->tl_valid && c->tl_bb == bb && c->tl_be == be && c->tl_form == p->form && same_schedule(c->tl_sched, sc))) {
+        tl = TileLists();
+        build_tiles(c->d, bb, be, p->form, sc, tl);
+        if ((rc = upload_tiles(c, tl))) return rc;
+        c->tl_valid = true;
+        c->tl_bb = bb;
+        c->tl_be = be;
+        c->tl_form = p->form;
+        c->tl_sched = sc;
+    }
     HIPCHK(c, hipMemsetAsync(c->nhits, 0, sizeof(unsigned long long), c->stream));
     DumpSink ds;
     memset(&ds, 0, sizeof(ds));
@@ -1033,6 +1048,7 @@ int epi_set_genotypes(epi_ctx *c, const int32_t *words, int32_t nid, int32_t nsn
     if (!c) return EPI_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
     c->last_nhits = 0;
+    c->tl_valid = false;
     return ds_set_genotypes(c, c->d, words, nid, nsnp);
 }
 
@@ -1046,6 +1062,7 @@ int epi_set_phenotypes(epi_ctx *c, const float *phen, int32_t nphen, int32_t nid
 int epi_set_chromosomes(epi_ctx *c, const int32_t *chrstart, const int32_t *chrsize, int32_t nchr)
 {
     if (!c) return EPI_ERR_ARG;
+    c->tl_valid = false;
     return ds_set_chromosomes(c, c->d, chrstart, chrsize, nchr);
 }
 

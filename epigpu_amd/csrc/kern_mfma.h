@@ -10,11 +10,13 @@
 // exact, so counts are bit-exact and cell sums are exact integers in units of
 // the fixed-point phenotype.
 //
-// Operands: class masks are stored as bytes 0xFF / 0x00 (int8 -1 / 0) in HBM,
-// [snp][plane][Kp], and staged per 128-individual chunk into LDS. An A operand
-// row is (a, g, w): mask(a, g) AND W_w. A B operand is the raw mask of 32
-// b-SNPs for one class, so every product carries a factor -1 that the
-// epilogue removes. The row order of the A operand is chosen so that the
+// Operands: class masks are stored as bitplanes in HBM ([plane][word][snp],
+// bit k of word w = individual 32w + k, planes: class 0, class 1, call
+// present) and expanded to bytes on their way into LDS, 128 individuals per
+// stage: HBM and L2 carry one bit per individual, the MFMA reads bytes. An A
+// operand row is (a, g, w): mask(a, g) AND W_w, with A masks expanded to
+// 0xFF / 0x00. A B operand is the mask of 32 b-SNPs for one class, expanded
+// to 0x01 / 0x00. The row order of the A operand is chosen so that the
 // output rows of one a-SNP land in one lane of the 32x32 accumulator
 // (lane l holds column l%32 and rows (i/4)*8 + (l/32)*4 + i%4), which puts a
 // whole pair table in one lane's registers.
@@ -35,6 +37,7 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 constexpr int KC = 128;          // individuals per LDS stage
 constexpr int ROWB = KC + 16;    // LDS row pitch in bytes (bank-conflict padding)
 constexpr int NW = 7;            // weight rows: V, L0, L1, L2, Z0, Z1, Z2
+constexpr int NWP = 8;           // rows in memory: the NW rows and an all-zero row
 
 // Identity class map: plane p holds class p (the MFMA form keeps genotype order).
 constexpr int CLS_IDENTITY = 0 | (1 << 2) | (2 << 4);
@@ -49,11 +52,11 @@ struct Marg {                    // per SNP, per phenotype, over present phenoty
 };
 
 struct MfmaScan {                // one dataset x one phenotype
-    const uint8_t *planes;       // [nsnp][3][Kp] masks
-    const int8_t *wts;           // [NW][Kp] for this phenotype
+    const uint32_t *bits;        // [3][Kw][nsnp] bitplanes: class 0, class 1, call present
+    const int8_t *wts;           // [NWP][Kp] for this phenotype
     const Marg *marg;            // [nsnp] for this phenotype
     int nsnp;
-    int Kp;
+    int Kp;                      // = 32 Kw
     int nV;                      // present phenotypes
     int tag;                     // written into the hits
     long long qV;                // sum q over present
@@ -83,6 +86,162 @@ __device__ __forceinline__ int xcd_tile(int bid, int n)
 
 __device__ __forceinline__ v4i lds_v4(const uint8_t *p) { return *reinterpret_cast<const v4i *>(p); }
 
+// 16 bits -> 16 bytes, bit j -> byte j, as 0x01 / 0x00 (B operands) or
+// 0xFF / 0x00 (A operands, which are ANDed with weight bytes). For a nibble
+// n, n * 0x00204081 = n + (n << 7) + (n << 14) + (n << 21) puts bit i of n
+// at bit 8i (the four shifted copies do not overlap), so masking with
+// 0x01010101 leaves one 0/1 per byte; (t << 8) - t widens each 1 to 0xFF.
+template <bool FF>
+__device__ __forceinline__ v4i expand16(uint32_t x)
+{
+    v4i r;
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+        const uint32_t t = (((x >> (4 * d)) & 0xFu) * 0x00204081u) & 0x01010101u;
+        r[d] = (int)(FF ? (t << 8) - t : t);
+    }
+    return r;
+}
+
+__device__ __forceinline__ v4i expand16_sel(uint32_t x, bool ff)
+{
+    v4i r;
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+        const uint32_t t = (((x >> (4 * d)) & 0xFu) * 0x00204081u) & 0x01010101u;
+        r[d] = (int)(ff ? (t << 8) - t : t);
+    }
+    return r;
+}
+
+// Staging of one tile's operands for a stage of KC individuals: NB dwords of
+// bitplanes per thread (A rows first, then B rows; thread-consecutive along
+// SNPs so that loads coalesce; each dword, 32 individuals of one SNP and
+// plane, becomes 32 bytes of one LDS row), and the NWR weight rows in 8-byte
+// pieces by the first NWR / 4 waves. Two register sets hold the stages in
+// flight, so that a stage's loads are issued two stages before it is written
+// to LDS. A rows are expanded to 0xFF bytes, B rows to 0x01 bytes.
+template <int NPL, int TA, int TB, int NB, int NWR>
+struct Stager {
+    static constexpr int NWD = KC / 32;                 // words per row per stage
+    static constexpr int NA = NPL * TA * NWD;           // A-row dwords per stage
+    static constexpr int NDW = NPL * (TA + TB) * NWD;   // dwords per stage
+    static constexpr int ROWS = NPL * (TA + TB);        // weight rows follow the SNP rows
+    static_assert(NB * 256 >= NDW && (NB - 1) * 256 < NDW, "dwords per thread");
+    static_assert(NWR % 4 == 0 && NWR * KC / 8 == 64 * (NWR / 4), "weight rows per wave");
+    const uint32_t *src[NB];
+    int dst[NB];
+    bool act[NB], isa[NB];
+    const int8_t *wsrc;
+    int wdst;
+    bool wload;
+    int nsnp;
+    uint32_t v[2][NB];
+    int2 w[2];
+    __device__ __forceinline__ Stager(const uint32_t *bits, const int8_t *wts, int Kp, int nsnp_, int a0, int b0,
+                                      int tid)
+    {
+        nsnp = nsnp_;
+        const int Kw = Kp / 32;
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            int c = tid + 256 * i;
+            act[i] = c < NDW;
+            c = c < NDW ? c : NDW - 1;
+            isa[i] = c < NA;
+            const int cc = isa[i] ? c : c - NA, T = isa[i] ? TA : TB;
+            const int sl = cc % T, pw = cc / T;
+            const int plane = pw / NWD, wd = pw % NWD;
+            int snp = isa[i] ? a0 + sl : b0 + sl;
+            snp = snp < nsnp ? snp : nsnp - 1;
+            src[i] = bits + ((size_t)plane * Kw + wd) * nsnp + snp;
+            const int row = isa[i] ? plane * TA + sl : NPL * TA + plane * TB + sl;
+            dst[i] = row * ROWB + wd * 32;
+        }
+        const int wv = tid >> 6, r = tid >> 4, ch = tid & 15;
+        wload = wv < NWR / 4;
+        wsrc = wts + (size_t)(wload ? r : 0) * Kp + ch * 8;
+        wdst = (ROWS + r) * ROWB + ch * 8;
+        w[0] = w[1] = make_int2(0, 0);
+    }
+    // stage starting at individual k0 (a multiple of KC) into register set S
+    template <int S>
+    __device__ __forceinline__ void load(int k0)
+    {
+        const size_t off = (size_t)(k0 / 32) * nsnp;
+#pragma unroll
+        for (int i = 0; i < NB; i++)
+            if (256 * (i + 1) <= NDW || act[i]) v[S][i] = src[i][off];
+        if (wload) w[S] = *reinterpret_cast<const int2 *>(wsrc + k0);
+    }
+    template <int S>
+    __device__ __forceinline__ void store(uint8_t *st) const
+    {
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            if (!(256 * (i + 1) <= NDW || act[i])) continue;
+            v4i lo, hi;
+            if (256 * (i + 1) <= NA) {
+                lo = expand16<true>(v[S][i] & 0xFFFFu);
+                hi = expand16<true>(v[S][i] >> 16);
+            } else if (256 * i >= NA) {
+                lo = expand16<false>(v[S][i] & 0xFFFFu);
+                hi = expand16<false>(v[S][i] >> 16);
+            } else {
+                lo = expand16_sel(v[S][i] & 0xFFFFu, isa[i]);
+                hi = expand16_sel(v[S][i] >> 16, isa[i]);
+            }
+            *reinterpret_cast<v4i *>(st + dst[i]) = lo;
+            *reinterpret_cast<v4i *>(st + dst[i] + 16) = hi;
+        }
+        if (wload) *reinterpret_cast<int2 *>(st + wdst) = w[S];
+    }
+};
+
+template <int S>
+struct SetTag {
+    static constexpr int v = S;
+};
+
+// The K loop of a tile over Kp individuals (a multiple of 2 KC) in stages
+// of KC through two LDS buffers. Within a stage the operands of the next
+// 32-individual step are read from LDS (fetch) while the MFMAs of the
+// current one run (issue); the next stage is written to the other buffer
+// mid-stage and one barrier per stage publishes it.
+template <class St, class Ops, class Fetch, class Issue>
+__device__ __forceinline__ void k_loop(St &sg, uint8_t *lds, int stage_bytes, int Kp, Fetch fetch, Issue issue)
+{
+    Ops o0, o1;
+    sg.template load<0>(0);
+    sg.template load<1>(KC);
+    sg.template store<0>(lds);
+    if (2 * KC < Kp) sg.template load<0>(2 * KC);
+    __syncthreads();
+    fetch(lds, 0, o0);
+    auto body = [&](auto tag, const uint8_t *cur, uint8_t *nxt, int kc) {
+        constexpr int S = decltype(tag)::v;
+        const bool more = kc + KC < Kp;
+        fetch(cur, 1, o1);
+        issue(o0);
+        fetch(cur, 2, o0);
+        if (more) {
+            sg.template store<S>(nxt);
+            if (kc + 3 * KC < Kp) sg.template load<S>(kc + 3 * KC);
+        }
+        issue(o1);
+        fetch(cur, 3, o1);
+        issue(o0);
+        // the next stage was written by every wave during this one
+        __syncthreads();
+        if (more) fetch(nxt, 0, o0);
+        issue(o1);
+    };
+    for (int k0 = 0; k0 < Kp; k0 += 2 * KC) {
+        body(SetTag<1>{}, lds, lds + stage_bytes, k0);
+        body(SetTag<0>{}, lds + stage_bytes, lds, k0 + KC);
+    }
+}
+
 // ---------------------------------------------------------------- FAST ----
 // Tile TA a x 64 b (TA = 64 or 32), 4 waves; wave w owns a-SNPs
 // [TA/4 w, TA/4 (w+1)) x all 64 b, as TA/16 blocks of 32 A rows.
@@ -100,32 +259,17 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
     constexpr int MB = TA / 16;               // 32-row A blocks per wave
     constexpr int APW = TA / 4;               // a-SNPs per wave
     constexpr int ROWS_AB = NPL * (TA + TB);  // 256 or 192
-    constexpr int NCH = ROWS_AB * 8 / 256;    // 16-byte chunks per thread per stage
+    constexpr int NCH = ROWS_AB * (KC / 32) / 256;  // bitplane dwords per thread per stage
     constexpr int STAGE = (ROWS_AB + 4) * ROWB;
     static_assert(2 * STAGE <= FAST_LDS, "LDS size");
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int Kp = A.Kp;
 
-    // Global -> LDS copy plan: ROWS_AB x 8 16-byte chunks of A/B rows, NCH per
-    // thread; 32 chunks of weight rows for threads 0..31.
-    const uint8_t *src[NCH];
-    int dst[NCH];
-#pragma unroll
-    for (int i = 0; i < NCH; i++) {
-        int c = tid + 256 * i, row = c >> 3, col = c & 7;
-        int side = row >= NPL * TA;                  // 0: A, 1: B
-        int r = side ? row - NPL * TA : row;
-        int plane = side ? r / TB : r / TA;
-        int sl = side ? r % TB : r % TA;
-        int snp = (side ? b0 : a0) + sl;
-        snp = snp < A.nsnp ? snp : A.nsnp - 1;
-        src[i] = A.planes + ((size_t)snp * 3 + plane) * (size_t)Kp + col * 16;
-        dst[i] = row * ROWB + col * 16;
-    }
-    const bool wload = tid < 32;
-    const int8_t *wsrc = A.wts + (size_t)(tid >> 3) * Kp + (tid & 7) * 16;
-    const int wdst = (ROWS_AB + (tid >> 3)) * ROWB + (tid & 7) * 16;
+    // Global -> LDS: ROWS_AB rows of bitplanes, expanded to bytes, and the 4
+    // weight rows V, L0, L1, L2.
+    Stager<NPL, TA, TB, NCH, 4> sg(A.bits, A.wts, Kp, A.nsnp, a0, b0, tid);
+    static_assert(NCH * 256 == Stager<NPL, TA, TB, NCH, 4>::NDW, "whole dwords per thread");
 
     // Operand geometry.
     const int m = lane & 31, hf = lane >> 5;
@@ -142,17 +286,6 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = (v16i){};
 
-    v4i buf[NCH], wbuf = {0, 0, 0, 0};
-    auto gload = [&](int k) {
-#pragma unroll
-        for (int i = 0; i < NCH; i++) buf[i] = *reinterpret_cast<const v4i *>(src[i] + k);
-        if (wload) wbuf = *reinterpret_cast<const v4i *>(wsrc + k);
-    };
-    auto lstore = [&](uint8_t *st) {
-#pragma unroll
-        for (int i = 0; i < NCH; i++) *reinterpret_cast<v4i *>(st + dst[i]) = buf[i];
-        if (wload) *reinterpret_cast<v4i *>(st + wdst) = wbuf;
-    };
     struct Ops {
         v4i w, b[4], am[MB];
     };
@@ -176,31 +309,7 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
         }
     };
 
-    Ops o0, o1;
-    gload(0);
-    lstore(lds);
-    if (KC < Kp) gload(KC);
-    __syncthreads();
-    fetch(lds, 0, o0);
-    for (int k0 = 0, s = 0; k0 < Kp; k0 += KC, s ^= 1) {
-        const uint8_t *cur = lds + s * STAGE;
-        uint8_t *nxt = lds + (s ^ 1) * STAGE;
-        const bool more = k0 + KC < Kp;
-        fetch(cur, 1, o1);
-        issue(o0);
-        fetch(cur, 2, o0);
-        if (more) {
-            lstore(nxt);
-            if (k0 + 2 * KC < Kp) gload(k0 + 2 * KC);
-        }
-        issue(o1);
-        fetch(cur, 3, o1);
-        issue(o0);
-        // the next stage was written by every wave during this one
-        __syncthreads();
-        if (more) fetch(nxt, 0, o0);
-        issue(o1);
-    }
+    k_loop<decltype(sg), Ops>(sg, lds, STAGE, Kp, fetch, issue);
 
     // Per-SNP class marginals of the tile, staged in LDS (the operand buffers
     // are free once every wave has left the K loop).
@@ -223,7 +332,7 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
     const float sqf = (float)A.sqV, qVf = (float)A.qV;
 
     // Epilogue: lane holds pairs (a = a0 + APW w + 4i + 2hf + p, b = b0 + 32bg + m);
-    // acc[i][2bg+h][8p + 4g + w] = -X[g][h][w].
+    // acc[i][2bg+h][8p + 4g + w] = X[g][h][w].
 #pragma unroll
     for (int i = 0; i < MB; i++) {
 #pragma unroll
@@ -242,8 +351,8 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
                     for (int h = 0; h < 2; h++) {
                         const v16i &v = acc[i][2 * bg + h];
                         const int e = 8 * p + 4 * g;
-                        xc[g][h] = -v[e];
-                        xf[g][h] = -((float)v[e + 1] + 256.0f * (float)v[e + 2] + 65536.0f * (float)v[e + 3]);
+                        xc[g][h] = v[e];
+                        xf[g][h] = (float)v[e + 1] + 256.0f * (float)v[e + 2] + 65536.0f * (float)v[e + 3];
                     }
                     xc[g][2] = mg_n[2 * al + g];
                     xf[g][2] = mg_f[2 * al + g];
@@ -265,7 +374,7 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
                         for (int h = 0; h < 2; h++) {
                             const v16i &v = acc[i][2 * bg + h];
                             const int e = 8 * p + 4 * g;
-                            xs[g][h] = -((double)v[e + 1] + 256.0 * (double)v[e + 2] + 65536.0 * (double)v[e + 3]);
+                            xs[g][h] = (double)v[e + 1] + 256.0 * (double)v[e + 2] + 65536.0 * (double)v[e + 3];
                         }
                         xs[g][2] = mg_q[2 * al + g];
                         xs[2][g] = mg_q[2 * bl + g];
@@ -310,40 +419,16 @@ __device__ __forceinline__ void mfma_general_tile(const MfmaScan &A, const HitSi
     static_assert(NBG == 1 || NBG == 2, "general tile width");
     constexpr int TA = 16, TB = 32 * NBG, NPL = 3;
     constexpr int ROWS_AB = NPL * (TA + TB);            // 240 or 144
-    constexpr int NCHUNK = (ROWS_AB + NW) * (KC / 16);  // 1976 or 1208
-    constexpr int NCH = (NCHUNK + 255) / 256;           // chunks per thread
+    constexpr int NCH = (NPL * (TA + TB) * (KC / 32) + 255) / 256;  // bitplane dwords per thread
     constexpr int STAGE = (ROWS_AB + NW + 1) * ROWB;    // + one zero row
     static_assert(2 * STAGE == general_lds<NBG>(), "LDS size");
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int Kp = A.Kp;
 
-    const uint8_t *src[NCH];
-    int dst[NCH];
-    bool act[NCH];
-#pragma unroll
-    for (int i = 0; i < NCH; i++) {
-        int c = tid + 256 * i, row = c >> 3, col = c & 7;
-        act[i] = c < NCHUNK;
-        if (row < ROWS_AB) {
-            int side = row >= NPL * TA;
-            int r = side ? row - NPL * TA : row;
-            int plane = side ? r / TB : r / TA;
-            int sl = side ? r % TB : r % TA;
-            int snp = (side ? b0 : a0) + sl;
-            snp = snp < A.nsnp ? snp : A.nsnp - 1;
-            src[i] = A.planes + ((size_t)snp * 3 + plane) * (size_t)Kp + col * 16;
-        } else {
-            int wr = row - ROWS_AB;
-            wr = wr < NW ? wr : NW - 1;
-            src[i] = reinterpret_cast<const uint8_t *>(A.wts) + (size_t)wr * Kp + col * 16;
-        }
-        dst[i] = row * ROWB + col * 16;
-    }
-    // zero row used by the padding A row, in both stage buffers
-    if (tid < 2 * (KC / 16))
-        *reinterpret_cast<v4i *>(lds + (tid >> 3) * STAGE + (ROWS_AB + NW) * ROWB + (tid & 7) * 16) =
-            (v4i){0, 0, 0, 0};
+    // Global -> LDS: ROWS_AB rows of bitplanes, expanded to bytes, and the NW
+    // weight rows plus the all-zero row NW that the padding A row reads.
+    Stager<NPL, TA, TB, NCH, NWP> sg(A.bits, A.wts, Kp, A.nsnp, a0, b0, tid);
 
     const int m = lane & 31, hf = lane >> 5;
     const int aj = (m >> 2) & 1;          // a-SNP within the operand
@@ -360,17 +445,6 @@ __device__ __forceinline__ void mfma_general_tile(const MfmaScan &A, const HitSi
 #pragma unroll
         for (int j = 0; j < 3 * NBG; j++) acc[i][j] = (v16i){};
 
-    v4i buf[NCH];
-    auto gload = [&](int k) {
-#pragma unroll
-        for (int i = 0; i < NCH; i++)
-            if (act[i]) buf[i] = *reinterpret_cast<const v4i *>(src[i] + k);
-    };
-    auto lstore = [&](uint8_t *st) {
-#pragma unroll
-        for (int i = 0; i < NCH; i++)
-            if (act[i]) *reinterpret_cast<v4i *>(st + dst[i]) = buf[i];
-    };
     struct Ops {
         v4i w, b[3 * NBG], am[2];
     };
@@ -394,33 +468,10 @@ __device__ __forceinline__ void mfma_general_tile(const MfmaScan &A, const HitSi
         }
     };
 
-    Ops o0, o1;
-    gload(0);
-    lstore(lds);
-    if (KC < Kp) gload(KC);
-    __syncthreads();
-    fetch(lds, 0, o0);
-    for (int k0 = 0, s = 0; k0 < Kp; k0 += KC, s ^= 1) {
-        const uint8_t *cur = lds + s * STAGE;
-        uint8_t *nxt = lds + (s ^ 1) * STAGE;
-        const bool more = k0 + KC < Kp;
-        fetch(cur, 1, o1);
-        issue(o0);
-        fetch(cur, 2, o0);
-        if (more) {
-            lstore(nxt);
-            if (k0 + 2 * KC < Kp) gload(k0 + 2 * KC);
-        }
-        issue(o1);
-        fetch(cur, 3, o1);
-        issue(o0);
-        __syncthreads();
-        if (more) fetch(nxt, 0, o0);
-        issue(o1);
-    }
+    k_loop<decltype(sg), Ops>(sg, lds, STAGE, Kp, fetch, issue);
 
     // Epilogue: lane holds pairs (a = a0 + 4w + 2i + hf, b = b0 + 32bg + m);
-    // acc[i][3bg+h][4g + w] = -X[g][h][w], g = 3 -> Z weights on NM.
+    // acc[i][3bg+h][4g + w] = X[g][h][w], g = 3 -> Z weights on NM.
 #pragma unroll
     for (int i = 0; i < 2; i++) {
         const int a = a0 + 4 * wid + 2 * i + hf;
@@ -434,11 +485,11 @@ __device__ __forceinline__ void mfma_general_tile(const MfmaScan &A, const HitSi
 #pragma unroll
                 for (int h = 0; h < 3; h++) {
                     const v16i &v = acc[i][3 * bg + h];
-                    xc[g][h] = -v[4 * g];
-                    xf[g][h] = -((float)v[4 * g + 1] + 256.0f * (float)v[4 * g + 2] + 65536.0f * (float)v[4 * g + 3]);
+                    xc[g][h] = v[4 * g];
+                    xf[g][h] = (float)v[4 * g + 1] + 256.0f * (float)v[4 * g + 2] + 65536.0f * (float)v[4 * g + 3];
                 }
             const v16i &vz = acc[i][3 * bg + 2];
-            const float sqf = -((float)vz[12] + 256.0f * (float)vz[13] + 65536.0f * (float)vz[14]) * 8388608.0f;
+            const float sqf = ((float)vz[12] + 256.0f * (float)vz[13] + 65536.0f * (float)vz[14]) * 8388608.0f;
             int n[9];
             float sf[9];
             cells_from_x(xc, n);
@@ -451,10 +502,10 @@ __device__ __forceinline__ void mfma_general_tile(const MfmaScan &A, const HitSi
 #pragma unroll
                     for (int h = 0; h < 3; h++) {
                         const v16i &v = acc[i][3 * bg + h];
-                        xs[g][h] = -((double)v[4 * g + 1] + 256.0 * (double)v[4 * g + 2] + 65536.0 * (double)v[4 * g + 3]);
+                        xs[g][h] = (double)v[4 * g + 1] + 256.0 * (double)v[4 * g + 2] + 65536.0 * (double)v[4 * g + 3];
                     }
                 cells_from_x(xs, sd);
-                sq = -((double)vz[12] + 256.0 * (double)vz[13] + 65536.0 * (double)vz[14]) * 8388608.0;
+                sq = ((double)vz[12] + 256.0 * (double)vz[13] + 65536.0 * (double)vz[14]) * 8388608.0;
             };
             const bool valid = a < a1 && b < b1 && a > b;
             finish_pair<DUMP>(hs, dsk, valid, a, b, n, sf, sqf, exact, CLS_IDENTITY, CLS_IDENTITY);

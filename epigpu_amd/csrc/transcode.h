@@ -19,36 +19,12 @@ __device__ __forceinline__ int decode(const uint32_t *w, int npack, int remain, 
     return (int)((w[j] >> sh) & 3u);
 }
 
-// One thread per (snp, 16-individual group): class masks for the MFMA form.
-__global__ void transcode_planes(const uint32_t *words, int nid, int npack, int remain, int npack1,
-                                 int nsnp, int Kp, uint8_t *planes, int32_t *miss)
-{
-    const int groups = Kp / 16;
-    long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= (long long)groups * nsnp) return;
-    int s = (int)(gid / groups), grp = (int)(gid % groups);
-    const uint32_t *w = words + (size_t)s * npack1;
-    uint32_t m[3][4] = {{0}};
-    int anymiss = 0;
-    for (int t = 0; t < 16; t++) {
-        int c = decode(w, npack, remain, nid, 16 * grp + t);
-        uint32_t byte = 0xFFu << (8 * (t & 3));
-        if (c == 0) m[0][t >> 2] |= byte;
-        if (c == 1) m[1][t >> 2] |= byte;
-        if (c >= 0 && c != 3) m[2][t >> 2] |= byte;
-        anymiss |= c == 3;
-    }
-    for (int p = 0; p < 3; p++)
-        *reinterpret_cast<uint4 *>(planes + ((size_t)s * 3 + p) * Kp + 16 * grp) =
-            make_uint4(m[p][0], m[p][1], m[p][2], m[p][3]);
-    if (anymiss) atomicOr(&miss[s], 1);
-}
-
-// One thread per SNP: order the genotype classes by frequency. The plain form
-// stores the two rarest classes as bitplanes 0 and 1 and derives the most
-// common one; cls packs plane -> class (2 bits each).
+// One thread per SNP: order the genotype classes by frequency and flag SNPs
+// with missing calls. The plain form stores the two rarest classes as
+// bitplanes 0 and 1 and derives the most common one; cls packs plane -> class
+// (2 bits each).
 __global__ void class_order(const uint32_t *words, int nid, int npack, int remain, int npack1, int nsnp,
-                            int32_t *cls)
+                            int32_t *cls, int32_t *miss)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsnp) return;
@@ -61,9 +37,12 @@ __global__ void class_order(const uint32_t *words, int nid, int npack, int remai
     int p0 = big == 0 ? 1 : 0;
     int p1 = big == 2 ? 1 : 2;
     cls[s] = p0 | (p1 << 2) | (big << 4);
+    miss[s] = n[3] > 0;
 }
 
-// One thread per (word of 32 individuals, snp): class bitplanes, plain form.
+// One thread per (word of 32 individuals, snp): bitplanes [3][Kw][nsnp] of
+// the classes cls names for planes 0 and 1 (cls == nullptr: classes 0 and 1,
+// the MFMA form's planes) and of the call-present flag in plane 2.
 __global__ void transcode_bits(const uint32_t *words, int nid, int npack, int remain, int npack1, int nsnp,
                                int Kw, const int32_t *cls, uint32_t *bits)
 {
@@ -71,7 +50,7 @@ __global__ void transcode_bits(const uint32_t *words, int nid, int npack, int re
     int wd = blockIdx.y;
     if (s >= nsnp) return;
     const uint32_t *w = words + (size_t)s * npack1;
-    const int c0 = cls[s] & 3, c1 = (cls[s] >> 2) & 3;
+    const int c0 = cls ? cls[s] & 3 : 0, c1 = cls ? (cls[s] >> 2) & 3 : 1;
     uint32_t b0 = 0, b1 = 0, bn = 0;
     for (int t = 0; t < 32; t++) {
         int c = decode(w, npack, remain, nid, 32 * wd + t);
@@ -84,64 +63,77 @@ __global__ void transcode_bits(const uint32_t *words, int nid, int npack, int re
     bits[((size_t)2 * Kw + wd) * nsnp + s] = bn;
 }
 
-// One workgroup per SNP: per genotype class (0, 1, 2) counts and phenotype
-// sums over present phenotypes, for one phenotype; stored per class for the
-// MFMA form and per bitplane (cls order) for the plain form.
-__global__ __launch_bounds__(256) void marginals_kernel(const uint8_t *planes, int Kp, int nsnp,
-                                                        const int8_t *wts, const int32_t *qv,
+// Per SNP, over individuals with a call and a present phenotype: counts and
+// phenotype sums of genotype classes 0, 1 and of all calls (NM), stored per
+// class for the MFMA form and per bitplane (cls order) for the plain form.
+// One workgroup per 64 SNPs: lane = SNP (coalesced bitplane loads), four
+// groups of threads split the words and are reduced in LDS.
+__global__ __launch_bounds__(256) void marginals_kernel(const uint32_t *mbits, int Kw, int nsnp,
+                                                        const uint32_t *vbits, const int32_t *qv,
                                                         const float *yc, const int32_t *cls, Marg *marg)
 {
-    __shared__ long long sq[3][256];
-    __shared__ double sy[3][256];
-    __shared__ int sn[3][256];
-    const int s = blockIdx.x, tid = threadIdx.x;
+    __shared__ long long sq[4][3][64];
+    __shared__ double sy[4][3][64];
+    __shared__ int sn[4][3][64];
+    const int sl = threadIdx.x & 63, grp = threadIdx.x >> 6;
+    const int s = blockIdx.x * 64 + sl;
     int n[3] = {0, 0, 0};
     long long q[3] = {0, 0, 0};
     double y[3] = {0.0, 0.0, 0.0};
-    const uint8_t *pl = planes + (size_t)s * 3 * Kp;
-    for (int k = tid; k < Kp; k += 256) {
-        if (!wts[k]) continue;  // row 0 = V
-        if (!pl[(size_t)2 * Kp + k]) continue;  // missing call
-        int c = pl[k] ? 0 : (pl[(size_t)Kp + k] ? 1 : 2);
-        n[c] += 1;
-        q[c] += qv[k];
-        y[c] += (double)yc[k];
+    if (s < nsnp) {
+        for (int wd = grp; wd < Kw; wd += 4) {
+            const uint32_t v = vbits[wd];
+            const uint32_t b0 = mbits[((size_t)0 * Kw + wd) * nsnp + s] & v;
+            const uint32_t b1 = mbits[((size_t)1 * Kw + wd) * nsnp + s] & v;
+            const uint32_t bn = mbits[((size_t)2 * Kw + wd) * nsnp + s] & v;
+            const uint32_t m[3] = {b0, b1, bn & ~b0 & ~b1};
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                uint32_t x = m[c];
+                n[c] += __popc(x);
+                while (x) {
+                    const int k = 32 * wd + __ffs(x) - 1;
+                    x &= x - 1;
+                    q[c] += qv[k];
+                    y[c] += (double)yc[k];
+                }
+            }
+        }
     }
-    for (int p = 0; p < 3; p++) {
-        sn[p][tid] = n[p];
-        sq[p][tid] = q[p];
-        sy[p][tid] = y[p];
+    for (int c = 0; c < 3; c++) {
+        sn[grp][c][sl] = n[c];
+        sq[grp][c][sl] = q[c];
+        sy[grp][c][sl] = y[c];
     }
     __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-        if (tid < st)
-            for (int p = 0; p < 3; p++) {
-                sn[p][tid] += sn[p][tid + st];
-                sq[p][tid] += sq[p][tid + st];
-                sy[p][tid] += sy[p][tid + st];
-            }
-        __syncthreads();
-    }
-    if (tid == 0) {
-        Marg m;
+    if (grp == 0 && s < nsnp) {
+        int tn[3];
+        long long tq[3];
+        double ty[3];
+        for (int c = 0; c < 3; c++) {
+            tn[c] = sn[0][c][sl] + sn[1][c][sl] + sn[2][c][sl] + sn[3][c][sl];
+            tq[c] = sq[0][c][sl] + sq[1][c][sl] + sq[2][c][sl] + sq[3][c][sl];
+            ty[c] = ((sy[0][c][sl] + sy[1][c][sl]) + sy[2][c][sl]) + sy[3][c][sl];
+        }
+        Marg mg;
         const int cl = cls[s];
-        const int ntot = sn[0][0] + sn[1][0] + sn[2][0];
-        m.n[0] = sn[0][0];
-        m.n[1] = sn[1][0];
-        m.n[2] = ntot;
-        m.q[0] = sq[0][0];
-        m.q[1] = sq[1][0];
-        m.q[2] = sq[0][0] + sq[1][0] + sq[2][0];
+        const int ntot = tn[0] + tn[1] + tn[2];
+        mg.n[0] = tn[0];
+        mg.n[1] = tn[1];
+        mg.n[2] = ntot;
+        mg.q[0] = tq[0];
+        mg.q[1] = tq[1];
+        mg.q[2] = tq[0] + tq[1] + tq[2];
         for (int p = 0; p < 2; p++) {
             int c = (cl >> (2 * p)) & 3;
-            m.pn[p] = sn[c][0];
-            m.ys[p] = sy[c][0];
+            mg.pn[p] = tn[c];
+            mg.ys[p] = ty[c];
         }
-        m.pn[2] = ntot;
-        m.ys[2] = sy[0][0] + sy[1][0] + sy[2][0];
-        m.cls = cl;
-        m.pad = 0;
-        marg[s] = m;
+        mg.pn[2] = ntot;
+        mg.ys[2] = ty[0] + ty[1] + ty[2];
+        mg.cls = cl;
+        mg.pad = 0;
+        marg[s] = mg;
     }
 }
 
