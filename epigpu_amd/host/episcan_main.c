@@ -9,6 +9,8 @@
  *     -g p       set -I from a p-value: F^-1(1-p; 4, N-9) (extension)
  *     -d n       HIP device (default 0)
  *     -p n       permutation n: shuffle phenotypes with srand(n), as the reference
+ *     -p a-b     permutations a..b in one batched device launch, one output file
+ *                <output.txt>.perm<k> per permutation (extension)
  *     -P file    phenotype file, one value per individual
  *     -m plain|mfma  kernel form (default mfma) (extension)
  *     -i n, -s n accepted for compatibility and ignored
@@ -26,7 +28,7 @@ static void usage(void)
 {
     fprintf(stderr,
             "usage: episcan -A <data.epigpu> <output.txt> [-t f|i] [-F x] [-I x] [-f p] [-g p]\n"
-            "               [-d n] [-p n] [-P file] [-m plain|mfma]\n");
+            "               [-d n] [-p n | -p a-b] [-P file] [-m plain|mfma]\n");
 }
 
 static int all_digits(const char *s)
@@ -43,18 +45,6 @@ static const char *base_name(const char *p)
     for (; *p; p++)
         if (*p == '/' || *p == '\\') b = p + 1;
     return b;
-}
-
-/* Fisher-Yates with the C library rand() as the reference's permute()
- * (src/main.c:336-357), applied to the phenotype records. */
-static void permute_ped(int nid, ped *dat)
-{
-    for (int i = nid; i > 1; i--) {
-        int r = rand() % i;
-        ped t = dat[r];
-        dat[r] = dat[i - 1];
-        dat[i - 1] = t;
-    }
 }
 
 /* 1/2-coded phenotype check (src/readplink.c:118-147). */
@@ -80,6 +70,7 @@ int main(int argc, char **argv)
     epi_run_opts_default(&o);
     const char *phenfile = "";
     double pf = -1.0, pi = -1.0;
+    int pfirst = 0, plast = 0;
     for (int i = 4; i < argc; i += 2) {
         const char *f = argv[i], *v = argv[i + 1];
         if (f[0] != '-' || !f[1] || f[2]) { usage(); return 1; }
@@ -93,7 +84,24 @@ int main(int argc, char **argv)
         case 'f': pf = atof(v); break;
         case 'g': pi = atof(v); break;
         case 'd': if (!all_digits(v)) { fprintf(stderr, "numeric -d required\n"); return 1; } o.device = atoi(v); break;
-        case 'p': if (!all_digits(v)) { fprintf(stderr, "numeric -p required\n"); return 1; } o.perm = atoi(v); break;
+        case 'p': {
+            const char *dash = strchr(v, '-');
+            if (dash) {
+                char a[32];
+                size_t la = (size_t)(dash - v);
+                if (la == 0 || la >= sizeof a) { fprintf(stderr, "-p a-b: numeric a and b required\n"); return 1; }
+                memcpy(a, v, la);
+                a[la] = 0;
+                if (!all_digits(a) || !all_digits(dash + 1)) { fprintf(stderr, "-p a-b: numeric a and b required\n"); return 1; }
+                pfirst = atoi(a);
+                plast = atoi(dash + 1);
+                if (pfirst < 1 || plast < pfirst) { fprintf(stderr, "-p a-b needs 1 <= a <= b\n"); return 1; }
+            } else {
+                if (!all_digits(v)) { fprintf(stderr, "numeric -p required\n"); return 1; }
+                o.perm = atoi(v);
+            }
+            break;
+        }
         case 'P': phenfile = v; break;
         case 'm':
             if (strcmp(v, "plain") == 0) o.form = EPI_FORM_PLAIN;
@@ -129,17 +137,30 @@ int main(int argc, char **argv)
         fprintf(stderr, "Error: Binary phenotype!\nepiGPU can only use quantitative phenotypes for analysis.\n");
         return 1;
     }
-    if (o.perm > 0) {
-        srand((unsigned)o.perm);
-        permute_ped(c.nid, c.dat);
+    if (o.perm > 0 && pfirst == 0) {
+        float *ph = (float *)malloc(sizeof(float) * (size_t)c.nid);
+        if (!ph) { fprintf(stderr, "out of memory\n"); return 1; }
+        epi_permute_phen((unsigned)o.perm, c.nid, c.dat, ph);
+        for (int i = 0; i < c.nid; i++) c.dat[i].phen = ph[i];
+        free(ph);
     }
     if (pf > 0.0) o.t1 = (float)epi_f_isf(pf, 8.0, (double)c.nid - 9.0);
     if (pi > 0.0) o.t2 = (float)epi_f_isf(pi, 4.0, (double)c.nid - 9.0);
     o.datafile = base_name(bin);
     o.phenfile = phenfile;
     epi_run_report rep;
-    rc = epi_perform_scan(&o, c.geno, c.chrstat, c.dat, c.genmap, c.nid, c.nsnp, c.npack, c.remain, c.nchr, outfile,
-                          &rep);
+    if (pfirst > 0) {
+        char fmt[4096];
+        if (strchr(outfile, '%') || snprintf(fmt, sizeof fmt, "%s.perm%%d", outfile) >= (int)sizeof fmt) {
+            fprintf(stderr, "output name unusable for -p a-b\n");
+            return 1;
+        }
+        rc = epi_perform_scan_perms(&o, c.geno, c.chrstat, c.dat, c.genmap, c.nid, c.nsnp, c.npack, c.remain, c.nchr,
+                                    pfirst, plast, fmt, &rep);
+    } else {
+        rc = epi_perform_scan(&o, c.geno, c.chrstat, c.dat, c.genmap, c.nid, c.nsnp, c.npack, c.remain, c.nchr,
+                              outfile, &rep);
+    }
     epi_container_free(&c);
     if (rc) {
         fprintf(stderr, "episcan: %s (%d)\n", epih_strerror(rc), rc);

@@ -295,3 +295,151 @@ done:
     rep->wall_s = now_s() - t0;
     return rc;
 }
+
+void epi_permute_phen(unsigned seed, int nid, const ped *dat, float *phen_out)
+{
+    for (int i = 0; i < nid; i++) phen_out[i] = dat[i].phen;
+    srand(seed);
+    for (int i = nid; i > 1; i--) {
+        int r = rand() % i;
+        float t = phen_out[r];
+        phen_out[r] = phen_out[i - 1];
+        phen_out[i - 1] = t;
+    }
+}
+
+/* Output file name of permutation k from a format with one %d. */
+static int perm_name(char *buf, size_t n, const char *fmt, int k)
+{
+    const char *pc = strstr(fmt, "%d");
+    if (!pc || strstr(pc + 2, "%") || strchr(fmt, '%') != pc) return EPIH_ERR_ARG;
+    int m = snprintf(buf, n, "%.*s%d%s", (int)(pc - fmt), fmt, k, pc + 2);
+    return m > 0 && (size_t)m < n ? EPIH_OK : EPIH_ERR_ARG;
+}
+
+int epi_perform_scan_perms(const epi_run_opts *opt, const int32_t *geno, const chromosome *chrstat, const ped *dat,
+                           const map *genmap, int nid, int nsnp, int npack, int remain, int nchr, int first, int last,
+                           const char *fmt, epi_run_report *rep)
+{
+    double t0 = now_s();
+    epi_run_report local;
+    if (!rep) rep = &local;
+    memset(rep, 0, sizeof(*rep));
+    if (!opt || !geno || !chrstat || !dat || !genmap || !fmt || nid < 2 || nsnp < 2 || nchr < 1 || first < 1 ||
+        last < first)
+        return EPIH_ERR_ARG;
+    if (npack != nid / 16 || remain != nid % 16) return EPIH_ERR_ARG;
+    if (opt->test != 'f' && opt->test != 'i') return EPIH_ERR_ARG;
+    const int np = last - first + 1;
+    char name[4096];
+    for (int k = first; k <= last; k++) {
+        if (perm_name(name, sizeof name, fmt, k)) return EPIH_ERR_ARG;
+        FILE *f = fopen(name, "r");
+        if (f) {
+            fclose(f);
+            return EPIH_ERR_DONE;
+        }
+    }
+    float *phen = (float *)malloc(sizeof(float) * (size_t)nid * (size_t)np);
+    int32_t *cs = (int32_t *)malloc(sizeof(int32_t) * (size_t)nchr);
+    int32_t *cz = (int32_t *)malloc(sizeof(int32_t) * (size_t)nchr);
+    epi_ctx *ctx = NULL;
+    epi_batch *b = NULL;
+    epi_hit *hits = NULL;
+    int32_t *blk = NULL;
+    FILE *out = NULL;
+    int rc = EPIH_OK;
+    if (!phen || !cs || !cz) { rc = EPIH_ERR_IO; goto done; }
+    for (int k = first; k <= last; k++) epi_permute_phen((unsigned)k, nid, dat, phen + (size_t)(k - first) * nid);
+    for (int i = 0; i < nchr; i++) {
+        cs[i] = chrstat[i].chrstart;
+        cz[i] = chrstat[i].chrsize;
+    }
+    if ((rc = epi_open(opt->device, &ctx)) || (rc = epi_batch_create(ctx, &b))) goto done;
+    int32_t d = 0, s0 = 0;
+    if ((rc = epi_batch_add_dataset(b, geno, nid, nsnp, cs, cz, nchr, &d)) ||
+        (rc = epi_batch_add_phenotypes(b, d, phen, np, &s0))) {
+        if (!opt->quiet) fprintf(stderr, "episcan: %s\n", epi_last_error(ctx));
+        goto done;
+    }
+    epi_scan_params p;
+    epi_scan_params_default(&p);
+    p.test = opt->test;
+    p.form = opt->form;
+    p.t1 = opt->t1;
+    p.t2 = opt->t2;
+    if (!opt->quiet) {
+        printf("Scanning permutations %d..%d in one batched launch\n", first, last);
+        fflush(stdout);
+    }
+    if ((rc = epi_batch_scan(b, &p))) goto done;
+    int64_t tot = 0;
+    int32_t ovf = 0;
+    epi_batch_hit_count(b, -1, &tot, &ovf);
+    if (ovf) {
+        /* grow the shared device buffer and rescan */
+        if ((rc = epi_batch_set_hit_capacity(b, tot + 1024)) || (rc = epi_batch_scan(b, &p))) goto done;
+        epi_batch_hit_count(b, -1, &tot, &ovf);
+        if (ovf) { rep->overflow = 1; rc = EPI_ERR_NOMEM; goto done; }
+    }
+    {
+        double ms = 0.0;
+        int64_t pairs = 0, items = 0;
+        int32_t caps = 0, grid = 0;
+        epi_batch_stats(b, &ms, &pairs, &items, &caps, &grid);
+        rep->kernel_s = ms * 1e-3;
+        rep->pairs = pairs;
+    }
+    const int nblk = nchr * (nchr + 1) / 2;
+    for (int k = first; k <= last; k++) {
+        const int s = s0 + (k - first);
+        int64_t nh = 0, got = 0;
+        if ((rc = epi_batch_hit_count(b, s, &nh, NULL))) goto done;
+        free(hits);
+        free(blk);
+        hits = (epi_hit *)malloc(sizeof(epi_hit) * (size_t)(nh > 0 ? nh : 1));
+        blk = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nh > 0 ? nh : 1));
+        if (!hits || !blk) { rc = EPIH_ERR_IO; goto done; }
+        if ((rc = epi_batch_get_hits(b, s, hits, NULL, NULL, blk, nh, &got))) goto done;
+        epi_run_opts o = *opt;
+        o.perm = k;
+        perm_name(name, sizeof name, fmt, k);
+        if ((rc = write_header(name, &o, nid, nsnp))) goto done;
+        out = fopen(name, "a");
+        if (!out) { rc = EPIH_ERR_IO; goto done; }
+        int64_t t = 0;
+        for (int i = 0, bi = 0; i < nchr; i++)
+            for (int j = 0; j <= i; j++, bi++) {
+                int64_t t1 = t;
+                while (t1 < got && blk[t1] == bi) t1++;
+                fprintf(out, "# %d x %d : %d interactions\n", i, j, (int)(t1 - t));
+                char row[512];
+                for (; t < t1; t++) {
+                    const epi_hit *h = &hits[t];
+                    epi_format_row(row, sizeof row, opt->test, h->snp1, genmap[h->snp1].snpname, chrstat[i].chrname,
+                                   h->snp2, genmap[h->snp2].snpname, chrstat[j].chrname, h->df1, h->df2, h->F,
+                                   h->F2);
+                    fputs(row, out);
+                }
+            }
+        if (fclose(out) != 0) { out = NULL; rc = EPIH_ERR_IO; goto done; }
+        out = NULL;
+        rep->new_hits += got;
+        rep->total_hits += got;
+        rep->blocks_done += nblk;
+        if (!opt->quiet) printf("permutation %d: %lld interactions -> %s\n", k, (long long)got, name);
+    }
+    if (!opt->quiet)
+        printf("\nProgramme finished in %.1f s (device %.3f s)\n", now_s() - t0, rep->kernel_s);
+done:
+    if (out) fclose(out);
+    if (b) epi_batch_destroy(b);
+    if (ctx) epi_close(ctx);
+    free(hits);
+    free(blk);
+    free(phen);
+    free(cs);
+    free(cz);
+    rep->wall_s = now_s() - t0;
+    return rc;
+}

@@ -163,6 +163,10 @@ def _declare(name, L):
             "epi_perform_scan": (ctypes.c_int, [_p(RunOpts), vp, vp, vp, vp, ctypes.c_int, ctypes.c_int,
                                                 ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
                                                 _p(RunReport)]),
+            "epi_permute_phen": (None, [ctypes.c_uint, ctypes.c_int, vp, vp]),
+            "epi_perform_scan_perms": (ctypes.c_int, [_p(RunOpts), vp, vp, vp, vp, ctypes.c_int, ctypes.c_int,
+                                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                      ctypes.c_int, ctypes.c_char_p, _p(RunReport)]),
             "epi_resume": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, ctypes.c_int, f32, f32, _p(ResumeState)]),
             "epi_format_row": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
@@ -316,6 +320,34 @@ def perform_scan(words, nid, ped, mp, chrs, filename, test="f", t1=6.5, t2=10.5,
     ped, mp, chrs = (np.ascontiguousarray(x) for x in (ped, mp, chrs))
     rc = H.epi_perform_scan(ctypes.byref(o), _ptr(words), _ptr(chrs), _ptr(ped), _ptr(mp), nid, words.shape[0],
                             npack, remain, len(chrs), os.fsencode(filename), ctypes.byref(rep))
+    if rc:
+        raise EpiError(rc, H.epih_strerror(rc).decode())
+    return {f: getattr(rep, f) for f, _ in RunReport._fields_}
+
+
+def permute_phen(seed, ped):
+    """Phenotypes of permutation `seed` (the reference's permute() with srand(seed))."""
+    ped = np.ascontiguousarray(ped)
+    out = np.zeros(len(ped), dtype=np.float32)
+    epihost().epi_permute_phen(seed, len(ped), _ptr(ped), _ptr(out))
+    return out
+
+
+def perform_scan_perms(words, nid, ped, mp, chrs, fmt, first, last, test="f", t1=6.5, t2=10.5, form="mfma",
+                       device=0, datafile="data.epigpu", phenfile="", quiet=True):
+    """Permutations first..last in one batched launch; output file k = fmt % k."""
+    H = epihost()
+    o = RunOpts()
+    H.epi_run_opts_default(ctypes.byref(o))
+    o.test, o.t1, o.t2, o.device, o.form = ord(test), t1, t2, device, FORMS[form]
+    o.datafile, o.phenfile, o.quiet = datafile.encode(), phenfile.encode(), int(quiet)
+    rep = RunReport()
+    words = np.ascontiguousarray(words, dtype=np.int32)
+    npack, remain, _ = layout(nid)
+    ped, mp, chrs = (np.ascontiguousarray(x) for x in (ped, mp, chrs))
+    rc = H.epi_perform_scan_perms(ctypes.byref(o), _ptr(words), _ptr(chrs), _ptr(ped), _ptr(mp), nid,
+                                  words.shape[0], npack, remain, len(chrs), first, last, os.fsencode(fmt),
+                                  ctypes.byref(rep))
     if rc:
         raise EpiError(rc, H.epih_strerror(rc).decode())
     return {f: getattr(rep, f) for f, _ in RunReport._fields_}

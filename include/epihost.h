@@ -170,6 +170,25 @@ int epi_perform_scan(const epi_run_opts *opt, const int32_t *geno,
                      int remain, int nchr, const char *filename,
                      epi_run_report *rep);
 
+/* Phenotypes of permutation k: the reference's permute() (src/main.c:336-357)
+ * shuffles the ped records with srand(k) and rand() % i from i = nid down to
+ * 2 (src/main.c:309-313); phen_out receives the permuted records' phen. */
+void epi_permute_phen(unsigned seed, int nid, const ped *dat, float *phen_out);
+
+/* Permutations first..last (1 <= first <= last) scanned together: the
+ * reference runs one process per permutation (-p k, src/main.c:101-112,
+ * :309-331); here every permutation is one scan of a single batched device
+ * launch sharing the genotype bitplanes. Output file of permutation k is
+ * fmt with its one %d replaced by k, written complete in the format of
+ * epi_perform_scan with "Permutation: k" in its header. Existing output files
+ * are refused (EPIH_ERR_DONE); this mode does not resume. opt->perm is
+ * ignored. rep sums over all permutations. */
+int epi_perform_scan_perms(const epi_run_opts *opt, const int32_t *geno,
+                           const chromosome *chrstat, const ped *dat,
+                           const map *genmap, int nid, int nsnp, int npack,
+                           int remain, int nchr, int first, int last,
+                           const char *fmt, epi_run_report *rep);
+
 /* Resume state parsed from an existing output file
  * (src/EpiOpenCL.c:1187-1274, with the chr2 restart fixed: a resumed run
  * continues at block (chr1, chr2) and then scans every later block). */

@@ -144,3 +144,39 @@ def test_reference_main_dropin(tmp_path, dataset):
     ours = str(tmp_path / "ours.txt")
     E.perform_scan(words, NID, ped, mp, chrs, ours, test="i", t1=T1, t2=T2)
     assert open(out).read() == open(ours).read()
+
+
+def test_batched_permutations_equal_single_runs(tmp_path, dataset):
+    """Permutations 1..3 in one batched launch write the same files as three single runs of permutation k
+    (phenotypes shuffled as the reference's permute() with srand(k))."""
+    import epigpu_amd as E
+    from epigpu_amd import scan as S
+    words, y, ped, mp, chrs = dataset
+    fmt = str(tmp_path / "batch.perm%d")
+    rep = S.perform_scan_perms(words, NID, ped, mp, chrs, fmt, 1, 3, test="i", t1=T1, t2=T2)
+    assert rep["blocks_done"] == 3 * 6
+    for k in (1, 2, 3):
+        pk = ped.copy()
+        pk["phen"] = S.permute_phen(k, ped)
+        one = str(tmp_path / f"single{k}.txt")
+        E.perform_scan(words, NID, pk, mp, chrs, one, test="i", t1=T1, t2=T2, perm=k)
+        assert open(fmt % k).read() == open(one).read()
+        assert "Permutation: %d\n" % k in open(one).read()
+
+
+def test_cli_permutation_range(tmp_path, dataset):
+    import epigpu_amd as E
+    words, y, ped, mp, chrs = dataset
+    data = str(tmp_path / "data.epigpu")
+    E.write_container(data, words, NID, ped, mp, chrs)
+    out = str(tmp_path / "cli.txt")
+    exe = os.path.join(ROOT, "epigpu_amd", "lib", "episcan")
+    r = subprocess.run([exe, "-A", data, out, "-t", "i", "-F", str(T1), "-I", str(T2), "-p", "2-3"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    single = str(tmp_path / "single.txt")
+    r = subprocess.run([exe, "-A", data, single, "-t", "i", "-F", str(T1), "-I", str(T2), "-p", "3"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert open(out + ".perm3").read() == open(single).read()
+    assert os.path.exists(out + ".perm2") and not os.path.exists(out)

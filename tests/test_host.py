@@ -233,3 +233,40 @@ def test_plant_adds_effect_to_one_cell():
     d = y - y0
     assert np.allclose(d[~changed], 0)
     assert (d[changed] > 0.149).all()
+
+
+def test_permute_phen_is_the_reference_shuffle():
+    """Fisher-Yates with the C library's rand() after srand(k), from i = nid down to 2 (src/main.c:336-347),
+    restated here with libc's own srand/rand through ctypes."""
+    import ctypes.util
+    from epigpu_amd import scan as S
+    libc = ctypes.CDLL(ctypes.util.find_library("c"))
+    nid = 257
+    ped, _, _ = S.make_records(nid, 4, [4], np.arange(nid, dtype=np.float32) * 0.5 - 3.0)
+    for k in (1, 2, 17):
+        got = S.permute_phen(k, ped)
+        ref = (np.arange(nid, dtype=np.float32) * 0.5 - 3.0).tolist()
+        libc.srand(k)
+        for i in range(nid, 1, -1):
+            r = libc.rand() % i
+            ref[r], ref[i - 1] = ref[i - 1], ref[r]
+        assert got.tolist() == ref
+        assert sorted(got.tolist()) == sorted(ped["phen"].tolist())
+    assert not np.array_equal(S.permute_phen(1, ped), S.permute_phen(2, ped))
+
+
+def test_perform_scan_perms_refuses_existing_outputs(tmp_path):
+    """Batched permutations never overwrite or resume: an existing output file is refused before any device work."""
+    from epigpu_amd import scan as S
+    nid, sizes = 64, [20]
+    words = S.synth_genotypes(3, nid, 20)
+    ped, mp, chrs = S.make_records(nid, 20, sizes, S.synth_phenotype(3, 0, nid))
+    fmt = str(tmp_path / "out.perm%d")
+    open(fmt % 2, "w").write("x")
+    with pytest.raises(S.EpiError) as e:
+        S.perform_scan_perms(words, nid, ped, mp, chrs, fmt, 1, 3)
+    assert e.value.code == -105  # EPIH_ERR_DONE
+    with pytest.raises(S.EpiError):
+        S.perform_scan_perms(words, nid, ped, mp, chrs, str(tmp_path / "bad%s%d"), 1, 2)
+    with pytest.raises(S.EpiError):
+        S.perform_scan_perms(words, nid, ped, mp, chrs, fmt, 3, 1)
