@@ -412,11 +412,13 @@ PlainScan plain_scan(const Dataset &d, int r, int tag)
 }
 
 // Schedules. FAST tiles (no missing call in either SNP range) are
-// fast_rows x 64. GENERAL tiles are 16 x 64, except 16 x 32 in the MFMA form
-// with 32-row FAST tiles (so that both fit two workgroups per CU).
-constexpr int TILE_B = 64, GEN_ROWS = 16;
+// fast_rows x 64. GENERAL tiles are fast_rows/2 x 32 in the MFMA form (the
+// same workgroup size as its FAST tiles, two waves per SIMD) and 16 x 64 in
+// the plain form.
+constexpr int TILE_B = 64;
 
-int gen_cols(int form, const epi_schedule &s) { return form == EPI_FORM_MFMA && s.fast_rows == 32 ? 32 : 64; }
+int gen_rows(int form, const epi_schedule &s) { return form == EPI_FORM_MFMA ? s.fast_rows / 2 : 16; }
+int gen_cols(int form, const epi_schedule &) { return form == EPI_FORM_MFMA ? 32 : 64; }
 
 epi_schedule default_schedule(int form)
 {
@@ -446,7 +448,7 @@ bool same_schedule(const epi_schedule &x, const epi_schedule &y)
            x.order == y.order;
 }
 
-void emit_tile(const Dataset &d, int a0, int a1, int b0, int b1, int gcols, TileLists &out)
+void emit_tile(const Dataset &d, int a0, int a1, int b0, int b1, int grows, int gcols, TileLists &out)
 {
     auto anymiss = [&](int s0, int s1) { return d.missc[s1] - d.missc[s0] > 0; };
     if (a1 - 1 <= b0) return;  // no pair with a > b
@@ -454,8 +456,8 @@ void emit_tile(const Dataset &d, int a0, int a1, int b0, int b1, int gcols, Tile
         out.fast.push_back(make_int4(a0, a1, b0, b1));
         return;
     }
-    for (int x = a0; x < a1; x += GEN_ROWS) {
-        int x1 = std::min(a1, x + GEN_ROWS);
+    for (int x = a0; x < a1; x += grows) {
+        int x1 = std::min(a1, x + grows);
         for (int y = b0; y < b1; y += gcols) {
             int y1 = std::min(b1, y + gcols);
             if (x1 - 1 <= y) continue;
@@ -469,7 +471,7 @@ void emit_tile(const Dataset &d, int a0, int a1, int b0, int b1, int gcols, Tile
 // neighbouring workgroups share SNP data in L2.
 void build_tiles(const Dataset &d, int blk_begin, int blk_end, int form, const epi_schedule &sc, TileLists &out)
 {
-    const int TA = sc.fast_rows, TB = TILE_B, gcols = gen_cols(form, sc);
+    const int TA = sc.fast_rows, TB = TILE_B, grows = gen_rows(form, sc), gcols = gen_cols(form, sc);
     const int nchr = (int)d.chrstart.size();
     const int SA = sc.super_a, SB = sc.super_b;
     for (int i = 0; i < nchr; i++) {
@@ -490,7 +492,7 @@ void build_tiles(const Dataset &d, int blk_begin, int blk_end, int form, const e
                         for (int tb = sb; tb < std::min(ntb, sb + SB); tb++) {
                             int a0 = A0 + ta * TA, a1 = std::min(A1, a0 + TA);
                             int b0 = B0 + tb * TB, b1 = std::min(B1, b0 + TB);
-                            emit_tile(d, a0, a1, b0, b1, gcols, out);
+                            emit_tile(d, a0, a1, b0, b1, grows, gcols, out);
                         }
         }
     }
@@ -566,9 +568,9 @@ int launch_pairs(epi_ctx *c, const epi_scan_params *p, const epi_schedule &sc, c
             A.tiles = c->tiles;
             A.ntiles = nf;
             if (sc.fast_rows == 64)
-                hipLaunchKernelGGL((mfma_fast_kernel<64, DUMP>), dim3(nf), dim3(256), 0, c->stream, A);
+                hipLaunchKernelGGL((mfma_fast_kernel<64, DUMP>), dim3(nf), dim3(fast_threads<64>()), 0, c->stream, A);
             else
-                hipLaunchKernelGGL((mfma_fast_kernel<32, DUMP>), dim3(nf), dim3(256), 0, c->stream, A);
+                hipLaunchKernelGGL((mfma_fast_kernel<32, DUMP>), dim3(nf), dim3(fast_threads<32>()), 0, c->stream, A);
             HIPCHK(c, hipGetLastError());
             c->last_launches++;
         }
@@ -576,9 +578,11 @@ int launch_pairs(epi_ctx *c, const epi_scan_params *p, const epi_schedule &sc, c
             A.tiles = c->tiles + nf;
             A.ntiles = ng;
             if (sc.fast_rows == 64)
-                hipLaunchKernelGGL((mfma_general_kernel<2, DUMP>), dim3(ng), dim3(256), 0, c->stream, A);
+                hipLaunchKernelGGL((mfma_general_kernel<32, 1, DUMP>), dim3(ng), dim3(general_threads<32>()), 0,
+                                   c->stream, A);
             else
-                hipLaunchKernelGGL((mfma_general_kernel<1, DUMP>), dim3(ng), dim3(256), 0, c->stream, A);
+                hipLaunchKernelGGL((mfma_general_kernel<16, 1, DUMP>), dim3(ng), dim3(general_threads<16>()), 0,
+                                   c->stream, A);
             HIPCHK(c, hipGetLastError());
             c->last_launches++;
         }
@@ -793,14 +797,15 @@ int capture_batch(epi_batch *b, const epi_scan_params *p)
     B.qend = b->d_qend;
     B.hs = make_sink(b->hits, b->nhits, b->cap, p);
     const void *kfn = batch_kernel(p->form, b->work_sched);
+    const int nt = p->form == EPI_FORM_MFMA ? 8 * b->work_sched.fast_rows : 256;
     int per_cu = 1;
-    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, 256, 0));
+    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, nt, 0));
     const int grid = std::max(1, per_cu) * c->ncu;
     void *args[] = {&B};
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, c->stream, b->d_qpos, (const int *)b->d_qbegin, b->nhits);
     hipError_t le = hipGetLastError();
-    if (le == hipSuccess) le = hipLaunchKernel(kfn, dim3(grid), dim3(256), args, 0, c->stream);
+    if (le == hipSuccess) le = hipLaunchKernel(kfn, dim3(grid), dim3(nt), args, 0, c->stream);
     hipError_t ce = hipStreamEndCapture(c->stream, &b->graph);
     if (le != hipSuccess) return fail(c, EPI_ERR_HIP, "launch during capture: %s", hipGetErrorString(le));
     if (ce != hipSuccess) return fail(c, EPI_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ce));
@@ -1200,7 +1205,8 @@ int epi_pair_dump(epi_ctx *c, const epi_scan_params *p, int32_t a0, int32_t a1, 
     TileLists all;
     for (int x = a0; x < a1; x += TA)
         for (int y = b0; y < b1; y += TB)
-            emit_tile(c->d, x, std::min(a1, x + TA), y, std::min(b1, y + TB), gen_cols(p->form, sc), all);
+            emit_tile(c->d, x, std::min(a1, x + TA), y, std::min(b1, y + TB), gen_rows(p->form, sc),
+                      gen_cols(p->form, sc), all);
     if ((rc = upload_tiles(c, all))) return rc;
     long long np = (long long)(a1 - a0) * (b1 - b0);
     int32_t *dc = nullptr;

@@ -121,13 +121,13 @@ __device__ __forceinline__ v4i expand16_sel(uint32_t x, bool ff)
 // pieces by the first NWR / 4 waves. Two register sets hold the stages in
 // flight, so that a stage's loads are issued two stages before it is written
 // to LDS. A rows are expanded to 0xFF bytes, B rows to 0x01 bytes.
-template <int NPL, int TA, int TB, int NB, int NWR>
+template <int NPL, int TA, int TB, int NB, int NWR, int NT>
 struct Stager {
     static constexpr int NWD = KC / 32;                 // words per row per stage
     static constexpr int NA = NPL * TA * NWD;           // A-row dwords per stage
     static constexpr int NDW = NPL * (TA + TB) * NWD;   // dwords per stage
     static constexpr int ROWS = NPL * (TA + TB);        // weight rows follow the SNP rows
-    static_assert(NB * 256 >= NDW && (NB - 1) * 256 < NDW, "dwords per thread");
+    static_assert(NB * NT >= NDW && (NB - 1) * NT < NDW, "dwords per thread");
     static_assert(NWR % 4 == 0 && NWR * KC / 8 == 64 * (NWR / 4), "weight rows per wave");
     const uint32_t *src[NB];
     int dst[NB];
@@ -145,7 +145,7 @@ struct Stager {
         const int Kw = Kp / 32;
 #pragma unroll
         for (int i = 0; i < NB; i++) {
-            int c = tid + 256 * i;
+            int c = tid + NT * i;
             act[i] = c < NDW;
             c = c < NDW ? c : NDW - 1;
             isa[i] = c < NA;
@@ -171,7 +171,7 @@ struct Stager {
         const size_t off = (size_t)(k0 / 32) * nsnp;
 #pragma unroll
         for (int i = 0; i < NB; i++)
-            if (256 * (i + 1) <= NDW || act[i]) v[S][i] = src[i][off];
+            if (NT * (i + 1) <= NDW || act[i]) v[S][i] = src[i][off];
         if (wload) w[S] = *reinterpret_cast<const int2 *>(wsrc + k0);
     }
     template <int S>
@@ -179,12 +179,12 @@ struct Stager {
     {
 #pragma unroll
         for (int i = 0; i < NB; i++) {
-            if (!(256 * (i + 1) <= NDW || act[i])) continue;
+            if (!(NT * (i + 1) <= NDW || act[i])) continue;
             v4i lo, hi;
-            if (256 * (i + 1) <= NA) {
+            if (NT * (i + 1) <= NA) {
                 lo = expand16<true>(v[S][i] & 0xFFFFu);
                 hi = expand16<true>(v[S][i] >> 16);
-            } else if (256 * i >= NA) {
+            } else if (NT * i >= NA) {
                 lo = expand16<false>(v[S][i] & 0xFFFFu);
                 hi = expand16<false>(v[S][i] >> 16);
             } else {
@@ -243,23 +243,26 @@ __device__ __forceinline__ void k_loop(St &sg, uint8_t *lds, int stage_bytes, in
 }
 
 // ---------------------------------------------------------------- FAST ----
-// Tile TA a x 64 b (TA = 64 or 32), 4 waves; wave w owns a-SNPs
-// [TA/4 w, TA/4 (w+1)) x all 64 b, as TA/16 blocks of 32 A rows.
+// Tile TA a x 64 b (TA = 32 or 64) with TA / 8 waves; wave w owns a-SNPs
+// [8w, 8w+8) x all 64 b as two blocks of 32 A rows: 8 MFMAs per
+// 32-individual step and 128 accumulator registers per lane, so a SIMD runs
+// two waves (two 256-thread workgroups, or one of 512 threads, per CU).
 // LDS rows per stage: A [plane 0..1][TA a], B [plane 0..1][64 b],
 // W [V, L0, L1, L2]. Two stage buffers; operands of the next 32-individual
 // step are read from LDS while the MFMAs of the current step run.
-// TA = 64 holds 256 accumulator registers per lane (one wave per SIMD);
-// TA = 32 holds 128, so two workgroups fit on a CU.
+template <int TA>
+constexpr int fast_threads() { return 8 * TA; }
+
 template <int TA, bool DUMP>
 __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink &hs, const DumpSink &dsk, int a0,
                                                int a1, int b0, int b1, uint8_t *lds)
 {
     static_assert(TA == 64 || TA == 32, "fast tile height");
-    constexpr int TB = 64, NPL = 2;
-    constexpr int MB = TA / 16;               // 32-row A blocks per wave
-    constexpr int APW = TA / 4;               // a-SNPs per wave
-    constexpr int ROWS_AB = NPL * (TA + TB);  // 256 or 192
-    constexpr int NCH = ROWS_AB * (KC / 32) / 256;  // bitplane dwords per thread per stage
+    constexpr int TB = 64, NPL = 2, NT = fast_threads<TA>();
+    constexpr int MB = 2;                     // 32-row A blocks per wave
+    constexpr int APW = 8;                    // a-SNPs per wave
+    constexpr int ROWS_AB = NPL * (TA + TB);  // 192 or 256
+    constexpr int NCH = ROWS_AB * (KC / 32) / NT;  // bitplane dwords per thread per stage
     constexpr int STAGE = (ROWS_AB + 4) * ROWB;
     static_assert(2 * STAGE <= FAST_LDS, "LDS size");
 
@@ -268,8 +271,8 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
 
     // Global -> LDS: ROWS_AB rows of bitplanes, expanded to bytes, and the 4
     // weight rows V, L0, L1, L2.
-    Stager<NPL, TA, TB, NCH, 4> sg(A.bits, A.wts, Kp, A.nsnp, a0, b0, tid);
-    static_assert(NCH * 256 == Stager<NPL, TA, TB, NCH, 4>::NDW, "whole dwords per thread");
+    Stager<NPL, TA, TB, NCH, 4, NT> sg(A.bits, A.wts, Kp, A.nsnp, a0, b0, tid);
+    static_assert(NCH * NT == Stager<NPL, TA, TB, NCH, 4, NT>::NDW, "whole dwords per thread");
 
     // Operand geometry.
     const int m = lane & 31, hf = lane >> 5;
@@ -394,7 +397,7 @@ template <int TA>
 constexpr int fast_lds() { return 2 * (2 * (TA + 64) + 4) * ROWB; }
 
 template <int TA, bool DUMP>
-__global__ __launch_bounds__(256, 64 / TA) void mfma_fast_kernel(const MfmaArgs A)
+__global__ __launch_bounds__(fast_threads<TA>(), 64 / TA) void mfma_fast_kernel(const MfmaArgs A)
 {
     __shared__ __attribute__((aligned(16))) uint8_t lds[fast_lds<TA>()];
     const int4 T = A.tiles[xcd_tile(blockIdx.x, A.ntiles)];
@@ -404,31 +407,36 @@ __global__ __launch_bounds__(256, 64 / TA) void mfma_fast_kernel(const MfmaArgs 
 }
 
 // ------------------------------------------------------------- GENERAL ----
-// Tile 16 a x TB b (TB = 32 NBG = 64 or 32), 4 waves; wave w owns a-SNPs
-// [4w, 4w+4) x all TB b. A operand: 2 a-SNPs x 16 rows: (g in {0,1,NM}) x
-// (V, L0, L1, L2), then (NM) x (Z0, Z1, Z2), then a zero row. Lane half hf
-// holds a-SNP hf. NBG = 2 holds 192 accumulator registers per lane (one wave
-// per SIMD), NBG = 1 holds 96 (two workgroups per CU).
-template <int NBG>
-constexpr int general_lds() { return 2 * (3 * (16 + 32 * NBG) + NW + 1) * ROWB; }
+// Tile GA a x TB b (GA = 16 or 32, TB = 32 NBG = 32 or 64) with GA / 4
+// waves; wave w owns a-SNPs [4w, 4w+4) x all TB b. A operand: 2 a-SNPs x 16
+// rows: (g in {0,1,NM}) x (V, L0, L1, L2), then (NM) x (Z0, Z1, Z2), then a
+// zero row. Lane half hf holds a-SNP hf. NBG = 1 holds 96 accumulator
+// registers per lane (two waves per SIMD), NBG = 2 holds 192 (one).
+template <int GA, int NBG>
+constexpr int general_lds() { return 2 * (3 * (GA + 32 * NBG) + NWP) * ROWB; }
 
-template <int NBG, bool DUMP>
+template <int GA>
+constexpr int general_threads() { return 16 * GA; }
+
+template <int GA, int NBG, bool DUMP>
 __device__ __forceinline__ void mfma_general_tile(const MfmaScan &A, const HitSink &hs, const DumpSink &dsk, int a0,
                                                   int a1, int b0, int b1, uint8_t *lds)
 {
     static_assert(NBG == 1 || NBG == 2, "general tile width");
-    constexpr int TA = 16, TB = 32 * NBG, NPL = 3;
-    constexpr int ROWS_AB = NPL * (TA + TB);            // 240 or 144
-    constexpr int NCH = (NPL * (TA + TB) * (KC / 32) + 255) / 256;  // bitplane dwords per thread
+    static_assert(GA == 16 || GA == 32, "general tile height");
+    static_assert(GA == 16 || NBG == 1, "eight waves of 192 accumulators do not fit a CU");
+    constexpr int TA = GA, TB = 32 * NBG, NPL = 3, NT = general_threads<GA>();
+    constexpr int ROWS_AB = NPL * (TA + TB);
+    constexpr int NCH = (NPL * (TA + TB) * (KC / 32) + NT - 1) / NT;  // bitplane dwords per thread
     constexpr int STAGE = (ROWS_AB + NW + 1) * ROWB;    // + one zero row
-    static_assert(2 * STAGE == general_lds<NBG>(), "LDS size");
+    static_assert(2 * STAGE == general_lds<GA, NBG>(), "LDS size");
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int Kp = A.Kp;
 
     // Global -> LDS: ROWS_AB rows of bitplanes, expanded to bytes, and the NW
     // weight rows plus the all-zero row NW that the padding A row reads.
-    Stager<NPL, TA, TB, NCH, NWP> sg(A.bits, A.wts, Kp, A.nsnp, a0, b0, tid);
+    Stager<NPL, TA, TB, NCH, NWP, NT> sg(A.bits, A.wts, Kp, A.nsnp, a0, b0, tid);
 
     const int m = lane & 31, hf = lane >> 5;
     const int aj = (m >> 2) & 1;          // a-SNP within the operand
@@ -513,14 +521,14 @@ __device__ __forceinline__ void mfma_general_tile(const MfmaScan &A, const HitSi
     }
 }
 
-template <int NBG, bool DUMP>
-__global__ __launch_bounds__(256, 3 - NBG) void mfma_general_kernel(const MfmaArgs A)
+template <int GA, int NBG, bool DUMP>
+__global__ __launch_bounds__(general_threads<GA>(), (3 - NBG) * 16 / GA) void mfma_general_kernel(const MfmaArgs A)
 {
-    __shared__ __attribute__((aligned(16))) uint8_t lds[general_lds<NBG>()];
+    __shared__ __attribute__((aligned(16))) uint8_t lds[general_lds<GA, NBG>()];
     const int4 T = A.tiles[xcd_tile(blockIdx.x, A.ntiles)];
     HitSink hs = A.hs;
     hs.tag = A.S.tag;
-    mfma_general_tile<NBG, DUMP>(A.S, hs, A.ds, T.x, T.y, T.z, T.w, lds);
+    mfma_general_tile<GA, NBG, DUMP>(A.S, hs, A.ds, T.x, T.y, T.z, T.w, lds);
 }
 
 }  // namespace epi
