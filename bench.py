@@ -60,6 +60,38 @@ def cpu_baseline(wl, seconds, threads):
             "sample": f"{len(tl)} seeded 128x128 off-diagonal tiles of the workload ({npairs} pairs, {dt:.1f} s)"}
 
 
+# HBM bytes per launch of the dominant kernel, from the committed rocprofv3 summary of the same workload and
+# schedule (FETCH_SIZE + WRITE_SIZE, each scaled by its factor on a copy of known size).
+# Profiled at FAST tile height 32 with 16 x 2 super-tiles; other super-tile shapes of the same tile height
+# differ only in L2 reuse, and HBM runs at about 1% of its bandwidth either way.
+PROFILED = {("c3", "mfma", 32): "profiles/mfma_fast_kernel_c3.md"}
+
+
+def profiled_traffic(workload, form, sched):
+    path = PROFILED.get((workload, form, sched["fast_rows"]))
+    if not path or not os.path.exists(os.path.join(ROOT, path)):
+        return None
+    for line in open(os.path.join(ROOT, path)):
+        if line.startswith("TRAFFIC_BYTES_PER_LAUNCH"):
+            return float(line.split()[1])
+    return None
+
+
+FP32_PEAK_TFLOPS = 157.3  # MI355X vector FP32
+
+
+def roofline(form, wl, kernel_ms, traffic=None, source=None):
+    """MFMA form: 36 N logical INT8 ops per pair (9 cells x N x (count + sum) MACs) against the dense INT8 peak.
+    Plain form: 18 N logical FP32 flops per pair (9 cells x N sum FMAs) against the vector FP32 peak."""
+    if form == "mfma":
+        ops, peak, bound, note = 36.0 * wl.nid * wl.pairs, INT8_PEAK_TOPS, "mfma", "INT8 ops; 36*N logical ops per pair"
+    else:
+        ops, peak, bound, note = 18.0 * wl.nid * wl.pairs, FP32_PEAK_TFLOPS, "valu", "FP32; 18*N logical flops per pair"
+    achieved = ops / (kernel_ms * 1e-3) / 1e12
+    return {"bound": bound, "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+            "traffic": traffic, "traffic_source": source, "ops_per_pair": ops / wl.pairs, "note": note}
+
+
 def accuracy(s, wl, form, phen=0):
     """One 64x64 block of pairs against the FP64 oracle."""
     import oracle as O
@@ -300,8 +332,7 @@ def main():
         found = sum((int(a), int(b)) in hs for a, b, _, _ in wl.planted_pairs)
 
     kernel_ms = float(np.mean(kms))
-    logical_ops = 36.0 * wl.nid * wl.pairs  # 2*9*N count ops + 2*9*N sum ops per pair
-    achieved = logical_ops / (kernel_ms * 1e-3) / 1e12
+    traffic = profiled_traffic(wl.name, args.form, s.schedule(args.form))
     out = {
         "metric": METRIC, "value": value, "unit": "pair-tests/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
@@ -312,9 +343,8 @@ def main():
                                f"MAF U{wl.maf}, genotype missing {wl.gmiss}",
                    "form": args.form, "pairs_per_step": wl.pairs},
         "kernel_ms_per_step": kernel_ms,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
-                     "frac": achieved / INT8_PEAK_TOPS, "traffic": None,
-                     "ops_per_pair": 36 * wl.nid, "note": "INT8 ops; 36*N logical ops per pair"},
+        "roofline": roofline(args.form, wl, kernel_ms, traffic,
+                             PROFILED.get((wl.name, args.form, s.schedule(args.form)["fast_rows"]))),
         "hits": int(r["nhits"]), "planted_found": found,
         "prep_ms": prep_s * 1e3,
     }
@@ -322,6 +352,7 @@ def main():
         r2 = step("plain")
         out["plain_kernel_ms"] = r2["kernel_ms"]
         out["plain_pairs_per_s"] = wl.pairs / (r2["kernel_ms"] * 1e-3)
+        out["plain_roofline"] = roofline("plain", wl, r2["kernel_ms"])
         out["mfma_over_plain"] = out["plain_kernel_ms"] / kernel_ms
         same = set(zip(r2["hits"]["snp1"].tolist(), r2["hits"]["snp2"].tolist())) == \
             set(zip(hits["snp1"].tolist(), hits["snp2"].tolist()))
