@@ -8,7 +8,7 @@
 //   per phenotype r:
 //     wts   [7][Kp] int8   V, 3 limbs of q, 3 limbs of round(q^2/2^23)
 //     qv    [Kp]    int32  fixed-point phenotype q (0 where missing)
-//     yc    [Kp]    float  centred phenotype (0 where missing)
+//     yc    [Kp]    float  centred phenotype scaled by 2^-ybits to |yc| < 1 (0 where missing)
 //     vbits [Kw]    uint32 phenotype-present bits
 //     marg  [nsnp]  Marg   per-SNP class counts and sums
 // Kp = nid rounded up to 256, Kw = Kp / 32; padding individuals are absent.
@@ -43,6 +43,7 @@ struct PhenDev {
     uint32_t *vbits = nullptr;
     Marg *marg = nullptr;
     int fbits = 0;  // q = round((y - centre) * 2^fbits)
+    int ybits = 0;  // plain form: yc = (y - centre) * 2^-ybits, |yc| < 1
     double centre = 0.0;
     int nV = 0;
     long long qV = 0;
@@ -282,6 +283,10 @@ int ds_set_phenotypes(epi_ctx *c, Dataset &d, const float *phen, int nphen, int 
         int f = 0;
         if (maxabs > 0.0) f = (int)std::floor(std::log2((double)QMAX / maxabs));
         while (maxabs > 0.0 && std::ldexp(maxabs, f) > (double)QMAX) f--;
+        // the plain form's FP32 values are scaled by a power of two to |yc| < 1, so that products of
+        // phenotypes far from unit scale neither overflow nor underflow FP32 (F is scale-free)
+        int e = 0;
+        if (maxabs > 0.0) std::frexp(maxabs, &e);
         std::fill(w.begin(), w.end(), 0);
         std::fill(qv.begin(), qv.end(), 0);
         std::fill(yc.begin(), yc.end(), 0.0f);
@@ -314,13 +319,14 @@ int ds_set_phenotypes(epi_ctx *c, Dataset &d, const float *phen, int nphen, int 
             qv[k] = (int32_t)q;
             qsum += q;
             sq += (double)q * (double)q;
-            float ycf = (float)dv;
+            float ycf = (float)std::ldexp(dv, -e);
             yc[k] = ycf;
             ys += (double)ycf;
             y2 += (double)ycf * (double)ycf;
             vb[k >> 5] |= 1u << (k & 31);
         }
         p.fbits = f;
+        p.ybits = e;
         p.centre = centre;
         p.nV = nv;
         p.qV = qsum;
@@ -1228,7 +1234,7 @@ int epi_pair_dump(epi_ctx *c, const epi_scan_params *p, int32_t a0, int32_t a1, 
     ds.b0 = b0;
     ds.b1 = b1;
     const PhenDev &ph = c->d.ph[p->phen];
-    ds.unit = p->form == EPI_FORM_MFMA ? std::ldexp(1.0, -ph.fbits) : 1.0;
+    ds.unit = p->form == EPI_FORM_MFMA ? std::ldexp(1.0, -ph.fbits) : std::ldexp(1.0, ph.ybits);
     ds.centre = ph.centre;
     // Pre-fill with the caller's buffers so that pairs with a <= b keep their values.
     int hrc = EPI_OK;

@@ -1,0 +1,132 @@
+"""GPU edge cases of the scan against the FP64 oracle: phenotype scales far from 1, constant and almost
+all-missing phenotypes, the smallest inputs, one-SNP chromosomes, and the 64-row MFMA tiles (512-thread
+workgroups) on the per-pair dump."""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import make_dataset, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL_F = 1e-5
+FORMS = ["plain", "mfma"]
+
+
+def dump_vs_oracle(scanner, words, y, nid, form, rect=None):
+    scanner.set_genotypes(words, nid)
+    scanner.set_phenotypes(y)
+    nsnp = words.shape[0]
+    a0, a1, b0, b1 = rect or (0, nsnp, 0, nsnp)
+    counts, sums, F, F2 = scanner.pair_dump(a0, a1, b0, b1, test="i", form=form)
+    aa, bb = np.meshgrid(np.arange(a0, a1), np.arange(b0, b1), indexing="ij")
+    sel = aa > bb
+    ref = O.pairs(O.unpack(words, nid), y, np.stack([aa[sel], bb[sel]], axis=1))
+    assert (counts[sel] == ref["counts"]).all()
+    return F[sel], F2[sel], sums[sel], ref
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("scale", [1e-30, 1e-6, 1e6, 1e30])
+def test_phenotype_scale(scanner, form, scale):
+    """F is scale-free: phenotypes near the ends of the FP32 range give the oracle's F (and the unit-scale F)."""
+    nid, nsnp = 400, 96
+    words, y = make_dataset(41, nid, nsnp, gmiss=0.01, pmiss=0.02)
+    ys = (y.astype(np.float64) * scale).astype(np.float32)
+    print(f"scale {scale:g} form={form}", flush=True)
+    F, F2, sums, ref = dump_vs_oracle(scanner, words, ys, nid, form)
+    ok = ref["df1"] >= 1
+    assert rel_err(F[ok], ref["F"][ok]).max() <= TOL_F
+    assert rel_err(F2[ok], ref["F2"][ok]).max() <= TOL_F
+    assert np.abs(sums - ref["sums"]).max() <= 1e-5 * scale * nid
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_constant_phenotype_gives_no_hits(scanner, form):
+    """All present phenotypes equal: SSW = 0, F is 0/0 in FP64 and no pair is a hit."""
+    nid, nsnp = 300, 80
+    words, _ = make_dataset(42, nid, nsnp)
+    y = np.full(nid, 2.5, dtype=np.float32)
+    y[::7] = np.nan
+    scanner.set_genotypes(words, nid)
+    scanner.set_phenotypes(y)
+    scanner.set_chromosomes([0], [nsnp])
+    g = scanner.scan(test="i", form=form, t1=0.5, t2=0.5)
+    assert g["nhits"] == 0
+    F, F2, sums, ref = dump_vs_oracle(scanner, words, y, nid, form)
+    assert np.isnan(ref["F"]).all() and np.isnan(F).all()
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_two_present_phenotypes(scanner, form):
+    """Only two individuals with a phenotype: d2 < 1 everywhere, no hits, counts exact."""
+    nid, nsnp = 200, 40
+    words, _ = make_dataset(43, nid, nsnp)
+    y = np.full(nid, np.nan, dtype=np.float32)
+    y[3], y[150] = 1.0, -2.0
+    scanner.set_genotypes(words, nid)
+    scanner.set_phenotypes(y)
+    scanner.set_chromosomes([0], [nsnp])
+    g = scanner.scan(test="f", form=form, t1=0.0)
+    assert g["nhits"] == 0 and g["pairs"] == nsnp * (nsnp - 1) // 2
+    dump_vs_oracle(scanner, words, y, nid, form)
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_smallest_inputs(scanner, form):
+    """nid = 2 individuals and 2 SNPs: one pair, one block, each individual in its own cell (d2 = 0: no hit)."""
+    import epigpu_amd as E
+    codes = np.array([[0, 2], [1, 0]], dtype=np.int8)
+    words = E.pack(codes)
+    y = np.array([1.0, 3.0], dtype=np.float32)
+    scanner.set_genotypes(words, 2)
+    scanner.set_phenotypes(y)
+    scanner.set_chromosomes([0], [2])
+    g = scanner.scan(test="f", form=form, t1=0.0)
+    assert g["pairs"] == 1 and g["nhits"] == 0
+    counts, sums, F, F2 = scanner.pair_dump(1, 2, 0, 1, test="i", form=form)
+    ref = O.pairs(O.unpack(words, 2), y, np.array([[1, 0]]))
+    assert counts[0, 0].tolist() == ref["counts"][0].tolist() == [0, 0, 1, 1, 0, 0, 0, 0, 0]
+    assert np.allclose(sums[0, 0], ref["sums"][0])
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_one_snp_chromosomes(scanner, form):
+    """Chromosomes of one SNP: empty diagonal blocks, block numbering as the reference."""
+    nid, sizes = 350, [1, 30, 1, 25]
+    nsnp = sum(sizes)
+    words, y = make_dataset(44, nid, nsnp, gmiss=0.01, pmiss=0.02)
+    starts = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    scanner.set_genotypes(words, nid)
+    scanner.set_phenotypes(y)
+    scanner.set_chromosomes(starts, sizes)
+    g = scanner.scan(test="i", form=form, t1=1.0, t2=1.0)
+    n_ref, ref = O.scan(O.unpack(words, nid), y, starts, sizes, "i", 1.0, 1.0)
+    gs = set(zip(g["hits"]["snp1"].tolist(), g["hits"]["snp2"].tolist()))
+    rs = set(zip(ref["snp1"].tolist(), ref["snp2"].tolist()))
+    for pr in gs ^ rs:
+        r = ref[(ref["snp1"] == pr[0]) & (ref["snp2"] == pr[1])]
+        assert len(r) == 0 or min(abs(r["F"][0] - 1.0), abs(r["F2"][0] - 1.0)) < 1e-5
+    assert g["pairs"] == nsnp * (nsnp - 1) // 2
+    rb = {(int(a), int(c)): int(k) for a, c, k in zip(ref["snp1"], ref["snp2"], ref["block"])}
+    for a, c, k in zip(g["hits"]["snp1"].tolist(), g["hits"]["snp2"].tolist(), g["block"].tolist()):
+        if (a, c) in rb:
+            assert rb[(a, c)] == k
+    assert (np.diff(g["block"]) >= 0).all()
+
+
+@pytest.mark.parametrize("nid", [300, 1000])
+def test_mfma_64_row_tiles_dump(scanner, nid):
+    """The 64-row FAST tiles (512-thread workgroups) and 32 x 32 GENERAL tiles on the per-pair dump."""
+    nsnp = 150
+    words, y = make_dataset(45 + nid, nid, nsnp, gmiss=0.004, pmiss=0.02)
+    default = scanner.schedule("mfma")
+    try:
+        scanner.set_schedule("mfma", fast_rows=64)
+        print(f"mfma 64-row dump nid={nid}", flush=True)
+        F, F2, sums, ref = dump_vs_oracle(scanner, words, y, nid, "mfma")
+        ok = ref["df1"] >= 1
+        assert rel_err(F[ok], ref["F"][ok]).max() <= TOL_F
+        assert rel_err(F2[ok], ref["F2"][ok]).max() <= TOL_F
+    finally:
+        scanner.set_schedule("mfma", **default)
