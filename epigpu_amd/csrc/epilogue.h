@@ -36,6 +36,7 @@ struct HitSink {
     float t1, t2;    // exact thresholds
     float t1s, t2s;  // screening thresholds (lowered)
     int32_t tag;     // written into every hit
+    int32_t safe;    // 1: test 'i' without the nfac > 6 condition (the reference's safe kernels)
 };
 
 struct DumpSink {
@@ -112,17 +113,17 @@ __device__ __forceinline__ void stats64(const int n[9], const S s[9], double sq,
     F2 = (ssi * 0.25) / msw;
 }
 
-__device__ __forceinline__ bool hit_rule(int test, double F, double F2, int nfac, int d1, int d2, float t1,
-                                         float t2)
+__device__ __forceinline__ bool hit_rule(int test, int safe, double F, double F2, int nfac, int d1, int d2,
+                                         float t1, float t2)
 {
     if (d1 < 1 || d2 < 1) return false;
     if (test == 'f') return F > (double)t1;
-    return (F > (double)t1 || F2 > (double)t2) && nfac > 6;
+    return (F > (double)t1 || F2 > (double)t2) && (safe || nfac > 6);
 }
 
 // FP32 screen: false only if the pair certainly misses both thresholds.
-__device__ __forceinline__ bool screen32(const int n[9], const float s[9], float sq, int test, float t1s,
-                                         float t2s)
+__device__ __forceinline__ bool screen32(const int n[9], const float s[9], float sq, int test, int safe,
+                                         float t1s, float t2s)
 {
     int N = 0, nfac = 0;
     float tot = 0.0f;
@@ -134,7 +135,7 @@ __device__ __forceinline__ bool screen32(const int n[9], const float s[9], float
     }
     int d1 = nfac - 1, d2 = N - nfac;
     if (d1 < 1 || d2 < 1) return false;
-    if (test != 'f' && nfac <= 6) return false;
+    if (test != 'f' && !safe && nfac <= 6) return false;
     float rN = __builtin_amdgcn_rcpf((float)N);
     float mbar = tot * rN;
     float m[9];
@@ -205,7 +206,7 @@ __device__ __forceinline__ bool confirm64(const HitSink &hs, int a, int b, const
     h.F2 = hs.test == 'f' ? 0.0 : F2;
     h.tag = hs.tag;
     h.pad = 0;
-    return hit_rule(hs.test, F, F2, nfac, d1, d2, hs.t1, hs.t2);
+    return hit_rule(hs.test, hs.safe, F, F2, nfac, d1, d2, hs.t1, hs.t2);
 }
 
 // Finish one pair. n[9] are the exact cell counts; sf[9] and sqf are FP32
@@ -238,7 +239,7 @@ __device__ __forceinline__ void finish_pair(const HitSink &hs, const DumpSink &d
         }
         return;
     }
-    const bool cand = valid && screen32(n, sf, sqf, hs.test, hs.t1s, hs.t2s);
+    const bool cand = valid && screen32(n, sf, sqf, hs.test, hs.safe, hs.t1s, hs.t2s);
     bool hit = false;
     DevHit h;
     if (cand) {

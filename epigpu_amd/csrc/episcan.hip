@@ -1,7 +1,7 @@
 // libepiscan.so - C ABI over the MI355X (gfx950) pair-scan kernels.
 //
 // Device state of one dataset (a context holds one; a batch holds many):
-//   words   [nsnp][npack1] int32   genotypes as uploaded (layout A1)
+//   words   [nsnp][npack1] int32   genotypes as uploaded (layout A1), freed after the transcode
 //   mbits   [3][Kw][nsnp]  uint32  bitplanes of classes 0, 1 and call present (MFMA form)
 //   bits    [3][Kw][nsnp]  uint32  bitplanes of the two rarest classes and call present (plain form)
 //   cls     [nsnp]         int32   plain-form plane -> class map
@@ -235,6 +235,7 @@ int ds_set_genotypes(epi_ctx *c, Dataset &d, const int32_t *words, int nid, int 
     HIPCHK(c, hipMemcpyAsync(d.miss.data(), d.dmiss, (size_t)nsnp * sizeof(int32_t), hipMemcpyDeviceToHost,
                              c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(d.words);  // the uploaded words only feed the transcode
     d.missc.assign(nsnp + 1, 0);
     for (int s = 0; s < nsnp; s++) d.missc[s + 1] = d.missc[s] + (d.miss[s] != 0);
     d.marg_ok = false;
@@ -245,8 +246,10 @@ int ds_set_genotypes(epi_ctx *c, Dataset &d, const int32_t *words, int nid, int 
     return EPI_OK;
 }
 
-int ds_set_phenotypes(epi_ctx *c, Dataset &d, const float *phen, int nphen, int nid)
+int ds_set_phenotypes(epi_ctx *c, Dataset &d, const float *phen, int nphen, int nid, int flags)
 {
+    const bool nine_value = (flags & EPI_PHEN_NINE_IS_VALUE) != 0;
+    auto missing = [&](float v) { return v != v || (!nine_value && v == -9.0f); };
     if (!phen || nphen < 1 || nid < 2) return fail(c, EPI_ERR_ARG, "need phenotypes");
     if (d.nid && nid != d.nid) return fail(c, EPI_ERR_ARG, "nid %d differs from genotypes' nid %d", nid, d.nid);
     const int Kp = (nid + KPAD - 1) / KPAD * KPAD, Kw = Kp / 32;
@@ -268,7 +271,7 @@ int ds_set_phenotypes(epi_ctx *c, Dataset &d, const float *phen, int nphen, int 
         int nv = 0;
         for (int k = 0; k < nid; k++) {
             float v = y[k];
-            if (v != v || v == -9.0f) continue;
+            if (missing(v)) continue;
             if (std::isinf(v)) return fail(c, EPI_ERR_RANGE, "phenotype %d has an infinite value", r);
             sum += v;
             nv++;
@@ -277,7 +280,7 @@ int ds_set_phenotypes(epi_ctx *c, Dataset &d, const float *phen, int nphen, int 
         double centre = sum / nv, maxabs = 0.0;
         for (int k = 0; k < nid; k++) {
             float v = y[k];
-            if (v != v || v == -9.0f) continue;
+            if (missing(v)) continue;
             maxabs = std::max(maxabs, std::fabs((double)v - centre));
         }
         int f = 0;
@@ -295,7 +298,7 @@ int ds_set_phenotypes(epi_ctx *c, Dataset &d, const float *phen, int nphen, int 
         double sq = 0.0, ys = 0.0, y2 = 0.0;
         for (int k = 0; k < nid; k++) {
             float v = y[k];
-            if (v != v || v == -9.0f) continue;
+            if (missing(v)) continue;
             double dv = (double)v - centre;
             long long q = std::llround(std::ldexp(dv, f));
             if (q > QMAX || q < -QMAX) return fail(c, EPI_ERR_RANGE, "fixed-point phenotype out of range");
@@ -534,6 +537,7 @@ HitSink make_sink(DevHit *hits, unsigned long long *nhits, long long cap, const 
     hs.t1s = p->t1 * 0.98f;
     hs.t2s = p->t2 * 0.98f;
     hs.tag = p->phen;
+    hs.safe = p->safe ? 1 : 0;
     return hs;
 }
 
@@ -769,7 +773,7 @@ int build_batch_work(epi_batch *b, int form, const epi_schedule &sched)
 
 bool same_key(const epi_scan_params &x, const epi_scan_params &y)
 {
-    return x.test == y.test && x.form == y.form && x.t1 == y.t1 && x.t2 == y.t2;
+    return x.test == y.test && x.form == y.form && x.t1 == y.t1 && x.t2 == y.t2 && (x.safe != 0) == (y.safe != 0);
 }
 
 void drop_graph(epi_batch *b)
@@ -1063,11 +1067,17 @@ int epi_set_genotypes(epi_ctx *c, const int32_t *words, int32_t nid, int32_t nsn
     return ds_set_genotypes(c, c->d, words, nid, nsnp);
 }
 
-int epi_set_phenotypes(epi_ctx *c, const float *phen, int32_t nphen, int32_t nid)
+int epi_set_phenotypes_ex(epi_ctx *c, const float *phen, int32_t nphen, int32_t nid, int32_t flags)
 {
     if (!c) return EPI_ERR_ARG;
+    if (flags & ~EPI_PHEN_NINE_IS_VALUE) return fail(c, EPI_ERR_ARG, "unknown phenotype flags");
     HIPCHK(c, hipSetDevice(c->device));
-    return ds_set_phenotypes(c, c->d, phen, nphen, nid);
+    return ds_set_phenotypes(c, c->d, phen, nphen, nid, flags);
+}
+
+int epi_set_phenotypes(epi_ctx *c, const float *phen, int32_t nphen, int32_t nid)
+{
+    return epi_set_phenotypes_ex(c, phen, nphen, nid, 0);
 }
 
 int epi_set_chromosomes(epi_ctx *c, const int32_t *chrstart, const int32_t *chrsize, int32_t nchr)
@@ -1327,13 +1337,20 @@ int epi_batch_add_dataset(epi_batch *b, const int32_t *words, int32_t nid, int32
 
 int epi_batch_add_phenotypes(epi_batch *b, int32_t dataset, const float *phen, int32_t nphen, int32_t *first_scan)
 {
+    return epi_batch_add_phenotypes_ex(b, dataset, phen, nphen, 0, first_scan);
+}
+
+int epi_batch_add_phenotypes_ex(epi_batch *b, int32_t dataset, const float *phen, int32_t nphen, int32_t flags,
+                                int32_t *first_scan)
+{
     if (!b) return EPI_ERR_ARG;
     epi_ctx *c = b->ctx;
+    if (flags & ~EPI_PHEN_NINE_IS_VALUE) return fail(c, EPI_ERR_ARG, "unknown phenotype flags");
     if (dataset < 0 || dataset >= (int)b->data.size()) return fail(c, EPI_ERR_ARG, "no such dataset");
     Dataset &d = *b->data[dataset];
     if (!d.ph.empty()) return fail(c, EPI_ERR_STATE, "dataset %d already has phenotypes", dataset);
     HIPCHK(c, hipSetDevice(c->device));
-    int rc = ds_set_phenotypes(c, d, phen, nphen, d.nid);
+    int rc = ds_set_phenotypes(c, d, phen, nphen, d.nid, flags);
     if (!rc) rc = ds_marginals(c, d);
     if (rc) return rc;
     if (first_scan) *first_scan = (int32_t)b->scans.size();

@@ -5,8 +5,8 @@
  * src/epiopencl.h:21 and reads the same run-configuration globals
  * (src/epiopencl.h:9-16, defaults src/main.c:244-249), so the reference's
  * analysis-mode main() links against this library in place of EpiOpenCL.c.
- * UCHUNKSIZE and USAFE have no meaning here (one launch per block, no
- * compatibility kernels) and are ignored. EPI_FORM selects the kernel form
+ * UCHUNKSIZE has no meaning here (one launch per block) and is ignored;
+ * USAFE selects the safe-mode semantics. EPI_FORM selects the kernel form
  * ("plain" or "mfma", default mfma). Returns the total hit count as the
  * reference does, or a negative error code.
  */
@@ -35,6 +35,7 @@ int perform_scan(int *geno, chromosome *chrstat, ped *dat, map *genmap, int nid,
     if (UTHRESHOLD2 != 0.0f) o.t2 = UTHRESHOLD2;
     o.device = UDEVICE;
     o.perm = UPERM;
+    o.safe = USAFE ? 1 : 0;
     o.datafile = plinkbin;
     o.phenfile = UPHEN;
     const char *form = getenv("EPI_FORM");

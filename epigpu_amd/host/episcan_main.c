@@ -13,7 +13,9 @@
  *                <output.txt>.perm<k> per permutation (extension)
  *     -P file    phenotype file, one value per individual
  *     -m plain|mfma  kernel form (default mfma) (extension)
- *     -i n, -s n accepted for compatibility and ignored
+ *     -s 0|1     safe mode: -9 is a phenotype value and test i drops nfac > 6, as
+ *                the reference's safe kernels (src/EpiOpenCL.c:505-963)
+ *     -i n       accepted for compatibility and ignored (tiles replace launch chunks)
  * Options come in pairs, as in the reference (src/main.c:63).
  */
 #include <ctype.h>
@@ -113,7 +115,7 @@ int main(int argc, char **argv)
             if (!all_digits(v) || k < 16 || k % 16) { fprintf(stderr, "Iteration size must be a multiple of 16\n"); return 1; }
             break;
         }
-        case 's': if (strcmp(v, "0") && strcmp(v, "1")) { fprintf(stderr, "-s 0|1\n"); return 1; } break;
+        case 's': if (strcmp(v, "0") && strcmp(v, "1")) { fprintf(stderr, "-s 0|1\n"); return 1; } o.safe = v[0] == '1'; break;
         default: usage(); return 1;
         }
     }

@@ -209,7 +209,8 @@ int epi_perform_scan(const epi_run_opts *opt, const int32_t *geno, const chromos
 
     /* device (src/EpiOpenCL.c:1474-1524) */
     if ((rc = epi_open(opt->device, &ctx))) goto done;
-    if ((rc = epi_set_genotypes(ctx, geno, nid, nsnp)) || (rc = epi_set_phenotypes(ctx, phen, 1, nid)) ||
+    if ((rc = epi_set_genotypes(ctx, geno, nid, nsnp)) ||
+        (rc = epi_set_phenotypes_ex(ctx, phen, 1, nid, opt->safe ? EPI_PHEN_NINE_IS_VALUE : 0)) ||
         (rc = epi_set_chromosomes(ctx, cs, cz, nchr))) {
         if (!opt->quiet) fprintf(stderr, "episcan: %s\n", epi_last_error(ctx));
         goto done;
@@ -226,6 +227,7 @@ int epi_perform_scan(const epi_run_opts *opt, const int32_t *geno, const chromos
     p.form = opt->form;
     p.t1 = opt->t1;
     p.t2 = opt->t2;
+    p.safe = opt->safe;
     int64_t hcap = 0;
     for (int i = 0, blk = 0; i < nchr; i++) {
         for (int j = 0; j <= i; j++, blk++) {
@@ -358,7 +360,7 @@ int epi_perform_scan_perms(const epi_run_opts *opt, const int32_t *geno, const c
     if ((rc = epi_open(opt->device, &ctx)) || (rc = epi_batch_create(ctx, &b))) goto done;
     int32_t d = 0, s0 = 0;
     if ((rc = epi_batch_add_dataset(b, geno, nid, nsnp, cs, cz, nchr, &d)) ||
-        (rc = epi_batch_add_phenotypes(b, d, phen, np, &s0))) {
+        (rc = epi_batch_add_phenotypes_ex(b, d, phen, np, opt->safe ? EPI_PHEN_NINE_IS_VALUE : 0, &s0))) {
         if (!opt->quiet) fprintf(stderr, "episcan: %s\n", epi_last_error(ctx));
         goto done;
     }
@@ -368,6 +370,7 @@ int epi_perform_scan_perms(const epi_run_opts *opt, const int32_t *geno, const c
     p.form = opt->form;
     p.t1 = opt->t1;
     p.t2 = opt->t2;
+    p.safe = opt->safe;
     if (!opt->quiet) {
         printf("Scanning permutations %d..%d in one batched launch\n", first, last);
         fflush(stdout);

@@ -15,6 +15,7 @@ LIBDIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
 FORM_PLAIN = 0
 FORM_MFMA = 1
 FORMS = {"plain": FORM_PLAIN, "mfma": FORM_MFMA}
+PHEN_NINE_IS_VALUE = 1
 
 PHEN_NORMAL, PHEN_GAMMA2, PHEN_GAMMA2_STD, PHEN_T5, PHEN_BERN02 = range(5)
 
@@ -38,7 +39,7 @@ HIT_DTYPE = np.dtype([("snp1", "<i4"), ("snp2", "<i4"), ("df1", "<i4"), ("df2", 
 class ScanParams(ctypes.Structure):
     _fields_ = [("test", ctypes.c_int32), ("form", ctypes.c_int32), ("t1", ctypes.c_float),
                 ("t2", ctypes.c_float), ("phen", ctypes.c_int32), ("blk_begin", ctypes.c_int32),
-                ("blk_end", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4)]
+                ("blk_end", ctypes.c_int32), ("safe", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
 
 
 class Schedule(ctypes.Structure):
@@ -59,7 +60,8 @@ TUNE_ROWS, TUNE_SUPER, TUNE_QUEUES, TUNE_ORDER, TUNE_ALL = 1, 2, 4, 8, 15
 class RunOpts(ctypes.Structure):
     _fields_ = [("test", ctypes.c_int), ("t1", ctypes.c_float), ("t2", ctypes.c_float),
                 ("device", ctypes.c_int), ("form", ctypes.c_int), ("perm", ctypes.c_int),
-                ("datafile", ctypes.c_char_p), ("phenfile", ctypes.c_char_p), ("quiet", ctypes.c_int)]
+                ("datafile", ctypes.c_char_p), ("phenfile", ctypes.c_char_p), ("quiet", ctypes.c_int),
+                ("safe", ctypes.c_int)]
 
 
 class RunReport(ctypes.Structure):
@@ -118,6 +120,7 @@ def _declare(name, L):
             "epi_device_name": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_size_t]),
             "epi_set_genotypes": (ctypes.c_int, [vp, vp, i32, i32]),
             "epi_set_phenotypes": (ctypes.c_int, [vp, vp, i32, i32]),
+            "epi_set_phenotypes_ex": (ctypes.c_int, [vp, vp, i32, i32, i32]),
             "epi_set_chromosomes": (ctypes.c_int, [vp, vp, vp, i32]),
             "epi_scan_params_default": (None, [_p(ScanParams)]),
             "epi_scan": (ctypes.c_int, [vp, _p(ScanParams)]),
@@ -130,6 +133,7 @@ def _declare(name, L):
             "epi_batch_destroy": (None, [vp]),
             "epi_batch_add_dataset": (ctypes.c_int, [vp, vp, i32, i32, vp, vp, i32, _p(i32)]),
             "epi_batch_add_phenotypes": (ctypes.c_int, [vp, i32, vp, i32, _p(i32)]),
+            "epi_batch_add_phenotypes_ex": (ctypes.c_int, [vp, i32, vp, i32, i32, _p(i32)]),
             "epi_batch_set_hit_capacity": (ctypes.c_int, [vp, i64]),
             "epi_batch_scan": (ctypes.c_int, [vp, _p(ScanParams)]),
             "epi_batch_hit_count": (ctypes.c_int, [vp, i32, _p(i64), _p(i32)]),
@@ -307,12 +311,13 @@ def write_container(path, words, nid, ped, mp, chrs):
 
 
 def perform_scan(words, nid, ped, mp, chrs, filename, test="f", t1=6.5, t2=10.5, form="mfma", device=0,
-                 datafile="data.epigpu", phenfile="", perm=0, quiet=True):
+                 datafile="data.epigpu", phenfile="", perm=0, quiet=True, safe=False):
     """Run the C host driver (mirror of the reference's perform_scan)."""
     H = epihost()
     o = RunOpts()
     H.epi_run_opts_default(ctypes.byref(o))
     o.test, o.t1, o.t2, o.device, o.form, o.perm = ord(test), t1, t2, device, FORMS[form], perm
+    o.safe = int(safe)
     o.datafile, o.phenfile, o.quiet = datafile.encode(), phenfile.encode(), int(quiet)
     rep = RunReport()
     words = np.ascontiguousarray(words, dtype=np.int32)
@@ -417,10 +422,11 @@ class Scanner:
         self._chk(self.L.epi_set_genotypes(self.ctx, _ptr(words), nid, words.shape[0]), "epi_set_genotypes")
         self.nid = nid
 
-    def set_phenotypes(self, phen):
+    def set_phenotypes(self, phen, nine_is_value=False):
+        """nine_is_value: -9 is a phenotype value, not missing (the reference's safe mode)."""
         phen = np.ascontiguousarray(np.atleast_2d(phen), dtype=np.float32)
-        self._chk(self.L.epi_set_phenotypes(self.ctx, _ptr(phen), phen.shape[0], phen.shape[1]),
-                  "epi_set_phenotypes")
+        self._chk(self.L.epi_set_phenotypes_ex(self.ctx, _ptr(phen), phen.shape[0], phen.shape[1],
+                                               PHEN_NINE_IS_VALUE if nine_is_value else 0), "epi_set_phenotypes")
 
     def set_chromosomes(self, starts, sizes):
         s = np.ascontiguousarray(starts, dtype=np.int32)
@@ -453,14 +459,15 @@ class Scanner:
                 "log": [dict(log[i].sched.as_dict(), ms=log[i].ms) for i in range(min(n.value, 64))]}
 
     @staticmethod
-    def params(test="f", form="mfma", t1=6.5, t2=10.5, phen=0, blk_begin=0, blk_end=0):
+    def params(test="f", form="mfma", t1=6.5, t2=10.5, phen=0, blk_begin=0, blk_end=0, safe=False):
         p = ScanParams()
         p.test, p.form, p.t1, p.t2, p.phen = ord(test), FORMS[form], t1, t2, phen
-        p.blk_begin, p.blk_end = blk_begin, blk_end
+        p.blk_begin, p.blk_end, p.safe = blk_begin, blk_end, int(safe)
         return p
 
-    def scan(self, test="f", form="mfma", t1=6.5, t2=10.5, phen=0, blk_begin=0, blk_end=0, fetch=True):
-        p = self.params(test, form, t1, t2, phen, blk_begin, blk_end)
+    def scan(self, test="f", form="mfma", t1=6.5, t2=10.5, phen=0, blk_begin=0, blk_end=0, fetch=True,
+             safe=False):
+        p = self.params(test, form, t1, t2, phen, blk_begin, blk_end, safe)
         self._chk(self.L.epi_scan(self.ctx, ctypes.byref(p)), "epi_scan")
         n = ctypes.c_int64()
         ovf = ctypes.c_int32()
@@ -542,19 +549,20 @@ class Batch:
                                                ctypes.byref(d)), "epi_batch_add_dataset")
         return d.value
 
-    def add_phenotypes(self, dataset, phen):
+    def add_phenotypes(self, dataset, phen, nine_is_value=False):
         phen = np.ascontiguousarray(np.atleast_2d(phen), dtype=np.float32)
         first = ctypes.c_int32()
-        self._chk(self.L.epi_batch_add_phenotypes(self.b, dataset, _ptr(phen), phen.shape[0], ctypes.byref(first)),
-                  "epi_batch_add_phenotypes")
+        self._chk(self.L.epi_batch_add_phenotypes_ex(self.b, dataset, _ptr(phen), phen.shape[0],
+                                                     PHEN_NINE_IS_VALUE if nine_is_value else 0,
+                                                     ctypes.byref(first)), "epi_batch_add_phenotypes")
         self.nscan = first.value + phen.shape[0]
         return list(range(first.value, first.value + phen.shape[0]))
 
     def set_hit_capacity(self, cap):
         self._chk(self.L.epi_batch_set_hit_capacity(self.b, cap), "epi_batch_set_hit_capacity")
 
-    def scan(self, test="f", form="mfma", t1=6.5, t2=10.5):
-        p = Scanner.params(test, form, t1, t2)
+    def scan(self, test="f", form="mfma", t1=6.5, t2=10.5, safe=False):
+        p = Scanner.params(test, form, t1, t2, safe=safe)
         self._chk(self.L.epi_batch_scan(self.b, ctypes.byref(p)), "epi_batch_scan")
         return self.stats()
 

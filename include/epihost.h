@@ -147,6 +147,8 @@ typedef struct epi_run_opts {
     const char *datafile;/* name written after "Datafile:" */
     const char *phenfile;/* second token of the Datafile line (-P), may be "" */
     int quiet;           /* 1: no progress lines on stdout */
+    int safe;            /* 1: the reference's safe mode (-s 1): -9 is a phenotype value and
+                            test 'i' drops nfac > 6 (src/EpiOpenCL.c:505-963, :1486-1499) */
 } epi_run_opts;
 void epi_run_opts_default(epi_run_opts *o);
 

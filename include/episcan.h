@@ -70,6 +70,15 @@ int epi_set_genotypes(epi_ctx *ctx, const int32_t *words, int32_t nid, int32_t n
  * the genotypes' nid when genotypes are set. */
 int epi_set_phenotypes(epi_ctx *ctx, const float *phen, int32_t nphen, int32_t nid);
 
+/* Phenotype flags. EPI_PHEN_NINE_IS_VALUE: -9 is an ordinary value and only
+ * NaN is missing, as in the reference's safe kernels, which test only the
+ * genotypes for missingness (src/EpiOpenCL.c:597-598). */
+#define EPI_PHEN_NINE_IS_VALUE 1
+
+/* As epi_set_phenotypes, with flags. */
+int epi_set_phenotypes_ex(epi_ctx *ctx, const float *phen, int32_t nphen, int32_t nid,
+                          int32_t flags);
+
 /* Chromosome blocks (chromosome.chrstart / chrsize, src/readplink.h:34-39).
  * Blocks are numbered in the reference's order: block(i, j) = i(i+1)/2 + j
  * for j <= i (src/EpiOpenCL.c:1527-1540, :1445-1456). */
@@ -83,7 +92,10 @@ typedef struct epi_scan_params {
     int32_t phen;       /* phenotype index */
     int32_t blk_begin;  /* first block to scan */
     int32_t blk_end;    /* one past the last block; <= 0 means all blocks */
-    int32_t reserved[4];/* zero */
+    int32_t safe;       /* 1: the reference's safe mode (-s 1, src/EpiOpenCL.c:505-963):
+                           test 'i' without the nfac > 6 condition (:949-950); load the
+                           phenotypes with EPI_PHEN_NINE_IS_VALUE for its -9 handling */
+    int32_t reserved[3];/* zero */
 } epi_scan_params;
 
 /* Fill defaults: test 'f', MFMA form, t1 6.5, t2 10.5 (src/main.c:244-249),
@@ -194,6 +206,10 @@ int epi_batch_add_dataset(epi_batch *batch, const int32_t *words, int32_t nid, i
  * each becomes one scan, numbered from *first_scan. Once per dataset. */
 int epi_batch_add_phenotypes(epi_batch *batch, int32_t dataset, const float *phen,
                              int32_t nphen, int32_t *first_scan);
+
+/* As epi_batch_add_phenotypes, with EPI_PHEN_* flags. */
+int epi_batch_add_phenotypes_ex(epi_batch *batch, int32_t dataset, const float *phen,
+                                int32_t nphen, int32_t flags, int32_t *first_scan);
 
 /* Device capacity of the batch's shared hit buffer (records); default 1<<22. */
 int epi_batch_set_hit_capacity(epi_batch *batch, int64_t cap);

@@ -36,9 +36,9 @@ def lib():
         L = ctypes.CDLL(path)
         vp, i32, i64, f32 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_float
         L.orc_unpack.argtypes = [vp, i32, i32, vp]
-        L.orc_pairs.argtypes = [vp, i32, vp, vp, i64, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_pairs.argtypes = [vp, i32, vp, i32, vp, i64, vp, vp, vp, vp, vp, vp, vp]
         L.orc_scan.restype = i64
-        L.orc_scan.argtypes = [vp, i32, vp, vp, vp, i32, i32, f32, f32, i32, i32, vp, i64]
+        L.orc_scan.argtypes = [vp, i32, vp, i32, vp, vp, i32, i32, f32, f32, i32, i32, vp, i64]
         L.ref32_pairs.argtypes = [vp, i32, vp, vp, i64, i32, f32, f32, vp]
         L.ref32_scan_tiles.restype = i64
         L.ref32_scan_tiles.argtypes = [vp, i32, i32, vp, vp, i32, i32, i32, f32, f32, ctypes.POINTER(i64)]
@@ -68,8 +68,8 @@ def unpack(words, nid):
     return codes
 
 
-def pairs(codes, phen, pairs_ab):
-    """FP64 per-pair statistics for an (n, 2) array of (a, b)."""
+def pairs(codes, phen, pairs_ab, safe=False):
+    """FP64 per-pair statistics for an (n, 2) array of (a, b); safe: -9 is a value (the reference's -s 1)."""
     codes = np.ascontiguousarray(codes, dtype=np.int8)
     phen = np.ascontiguousarray(phen, dtype=np.float32)
     pr = np.ascontiguousarray(pairs_ab, dtype=np.int32)
@@ -77,20 +77,20 @@ def pairs(codes, phen, pairs_ab):
     out = {"counts": np.zeros((n, 9), np.int32), "sums": np.zeros((n, 9), np.float64),
            "F": np.zeros(n, np.float64), "F2": np.zeros(n, np.float64), "df1": np.zeros(n, np.int32),
            "df2": np.zeros(n, np.int32), "nfac": np.zeros(n, np.int32)}
-    lib().orc_pairs(_p(codes), codes.shape[1], _p(phen), _p(pr), n, _p(out["counts"]), _p(out["sums"]),
+    lib().orc_pairs(_p(codes), codes.shape[1], _p(phen), int(safe), _p(pr), n, _p(out["counts"]), _p(out["sums"]),
                     _p(out["F"]), _p(out["F2"]), _p(out["df1"]), _p(out["df2"]), _p(out["nfac"]))
     return out
 
 
 def scan(codes, phen, chrstart, chrsize, test="f", t1=6.5, t2=10.5, blk_begin=0, blk_end=1 << 30,
-         cap=1 << 22):
-    """FP64 exhaustive scan; hits sorted by (block, snp1, snp2)."""
+         cap=1 << 22, safe=False):
+    """FP64 exhaustive scan; hits sorted by (block, snp1, snp2); safe: the reference's -s 1 semantics."""
     codes = np.ascontiguousarray(codes, dtype=np.int8)
     phen = np.ascontiguousarray(phen, dtype=np.float32)
     cs = np.ascontiguousarray(chrstart, dtype=np.int32)
     cz = np.ascontiguousarray(chrsize, dtype=np.int32)
     hits = np.zeros(cap, dtype=ORC_HIT_DTYPE)
-    n = lib().orc_scan(_p(codes), codes.shape[1], _p(phen), _p(cs), _p(cz), len(cs), ord(test), t1, t2,
+    n = lib().orc_scan(_p(codes), codes.shape[1], _p(phen), int(safe), _p(cs), _p(cz), len(cs), ord(test), t1, t2,
                        blk_begin, blk_end, _p(hits), cap)
     hits = hits[:min(n, cap)]
     hits = hits[np.lexsort((hits["snp2"], hits["snp1"], hits["block"]))]

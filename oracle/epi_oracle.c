@@ -23,6 +23,11 @@
  *   hit, test 'i': (F > t1 || F2 > t2) && nfac > 6              (:487-489)
  * F is only defined for d1 >= 1 and d2 >= 1; other pairs are never hits.
  *
+ * Safe mode (-s 1; kernels kernelfullsafe / kernelintsafe, src/EpiOpenCL.c:505-963,
+ * selected at :1486-1499): complete cases only require both calls, so -9 is an
+ * ordinary phenotype value (:597-598, no sentinel test), and test 'i' has no
+ * nfac > 6 condition (:949-950). NaN stays missing here in both modes.
+ *
  * Everything is computed from the raw genotypes and FP64 copies of the FP32
  * phenotype, two-pass, so the only rounding is FP64's.
  */
@@ -38,7 +43,7 @@ typedef struct orc_hit {
     int32_t pad;
 } orc_hit;
 
-static inline int y_missing(float y) { return y != y || y == -9.0f; }
+static inline int y_missing(float y, int safe) { return y != y || (!safe && y == -9.0f); }
 
 /* Decode layout A1 (src/readplink.c:1104-1143). */
 void orc_unpack(const int32_t *words, int nid, int nsnp, int8_t *codes)
@@ -56,7 +61,7 @@ void orc_unpack(const int32_t *words, int nid, int nsnp, int8_t *codes)
 }
 
 /* Cell counts and FP64 cell sums over complete cases. */
-void orc_pair_table(const int8_t *ga, const int8_t *gb, const float *y, int nid,
+void orc_pair_table(const int8_t *ga, const int8_t *gb, const float *y, int nid, int safe,
                     int64_t counts[9], double sums[9])
 {
     for (int c = 0; c < 9; c++) {
@@ -64,7 +69,7 @@ void orc_pair_table(const int8_t *ga, const int8_t *gb, const float *y, int nid,
         sums[c] = 0.0;
     }
     for (int k = 0; k < nid; k++) {
-        if (ga[k] == 3 || gb[k] == 3 || y_missing(y[k])) continue;
+        if (ga[k] == 3 || gb[k] == 3 || y_missing(y[k], safe)) continue;
         int c = 3 * ga[k] + gb[k];
         counts[c]++;
         sums[c] += (double)y[k];
@@ -72,11 +77,11 @@ void orc_pair_table(const int8_t *ga, const int8_t *gb, const float *y, int nid,
 }
 
 /* Full per-pair statistics (two-pass). Returns nfac. */
-int orc_pair_stats(const int8_t *ga, const int8_t *gb, const float *y, int nid,
+int orc_pair_stats(const int8_t *ga, const int8_t *gb, const float *y, int nid, int safe,
                    int64_t counts[9], double sums[9], double *F, double *F2,
                    int *d1, int *d2)
 {
-    orc_pair_table(ga, gb, y, nid, counts, sums);
+    orc_pair_table(ga, gb, y, nid, safe, counts, sums);
     int64_t n = 0;
     double S = 0.0;
     int nfac = 0;
@@ -96,7 +101,7 @@ int orc_pair_stats(const int8_t *ga, const int8_t *gb, const float *y, int nid,
     for (int c = 0; c < 9; c++) m[c] = counts[c] ? sums[c] / (double)counts[c] : 0.0;
     double ssw = 0.0;
     for (int k = 0; k < nid; k++) {
-        if (ga[k] == 3 || gb[k] == 3 || y_missing(y[k])) continue;
+        if (ga[k] == 3 || gb[k] == 3 || y_missing(y[k], safe)) continue;
         double e = (double)y[k] - m[3 * ga[k] + gb[k]];
         ssw += e * e;
     }
@@ -122,15 +127,15 @@ int orc_pair_stats(const int8_t *ga, const int8_t *gb, const float *y, int nid,
     return nfac;
 }
 
-static inline int is_hit(int test, double F, double F2, int nfac, int d1, int d2, float t1, float t2)
+static inline int is_hit(int test, int safe, double F, double F2, int nfac, int d1, int d2, float t1, float t2)
 {
     if (d1 < 1 || d2 < 1) return 0;
     if (test == 'f') return F > (double)t1;
-    return (F > (double)t1 || F2 > (double)t2) && nfac > 6;
+    return (F > (double)t1 || F2 > (double)t2) && (safe || nfac > 6);
 }
 
 /* Per-pair results for an explicit list of (a, b) pairs. */
-void orc_pairs(const int8_t *codes, int nid, const float *y, const int32_t *pairs,
+void orc_pairs(const int8_t *codes, int nid, const float *y, int safe, const int32_t *pairs,
                int64_t npairs, int32_t *counts_out, double *sums_out, double *F,
                double *F2, int32_t *d1, int32_t *d2, int32_t *nfac)
 {
@@ -139,7 +144,7 @@ void orc_pairs(const int8_t *codes, int nid, const float *y, const int32_t *pair
         int64_t cnt[9];
         double sm[9], f, f2;
         int a = pairs[2 * p], b = pairs[2 * p + 1], e1, e2;
-        int nf = orc_pair_stats(codes + (size_t)a * nid, codes + (size_t)b * nid, y, nid,
+        int nf = orc_pair_stats(codes + (size_t)a * nid, codes + (size_t)b * nid, y, nid, safe,
                                 cnt, sm, &f, &f2, &e1, &e2);
         for (int c = 0; c < 9; c++) {
             if (counts_out) counts_out[9 * p + c] = (int32_t)cnt[c];
@@ -159,7 +164,7 @@ void orc_pairs(const int8_t *codes, int nid, const float *y, const int32_t *pair
  * :1540, :1445-1456). Pairs a in chromosome i, b in chromosome j, a > b.
  * Returns the number of hits (all of them, also beyond cap).
  */
-int64_t orc_scan(const int8_t *codes, int nid, const float *y, const int32_t *chrstart,
+int64_t orc_scan(const int8_t *codes, int nid, const float *y, int safe, const int32_t *chrstart,
                  const int32_t *chrsize, int nchr, int test, float t1, float t2,
                  int blk_begin, int blk_end, orc_hit *hits, int64_t cap)
 {
@@ -177,8 +182,8 @@ int64_t orc_scan(const int8_t *codes, int nid, const float *y, const int32_t *ch
                     double sm[9], f, f2;
                     int e1, e2;
                     int nf = orc_pair_stats(codes + (size_t)a * nid, codes + (size_t)b * nid, y,
-                                            nid, cnt, sm, &f, &f2, &e1, &e2);
-                    if (!is_hit(test, f, f2, nf, e1, e2, t1, t2)) continue;
+                                            nid, safe, cnt, sm, &f, &f2, &e1, &e2);
+                    if (!is_hit(test, safe, f, f2, nf, e1, e2, t1, t2)) continue;
                     int64_t slot;
 #pragma omp atomic capture
                     slot = nh++;

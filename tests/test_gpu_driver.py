@@ -180,3 +180,28 @@ def test_cli_permutation_range(tmp_path, dataset):
     assert r.returncode == 0, r.stderr
     assert open(out + ".perm3").read() == open(single).read()
     assert os.path.exists(out + ".perm2") and not os.path.exists(out)
+
+
+def test_safe_mode_cli_and_reference_main(tmp_path, dataset):
+    """-s 1 through the CLI, perform_scan(safe=True) and the reference's own main() write the same file."""
+    import epigpu_amd as E
+    words, y, ped, mp, chrs = dataset
+    data = str(tmp_path / "data.epigpu")
+    E.write_container(data, words, NID, ped, mp, chrs)
+    ours = str(tmp_path / "ours.txt")
+    E.perform_scan(words, NID, ped, mp, chrs, ours, test="i", t1=T1, t2=T2, safe=True)
+    normal = str(tmp_path / "normal.txt")
+    E.perform_scan(words, NID, ped, mp, chrs, normal, test="i", t1=T1, t2=T2)
+    assert open(ours).read() != open(normal).read()
+    cli = str(tmp_path / "cli.txt")
+    r = subprocess.run([os.path.join(ROOT, "epigpu_amd", "lib", "episcan"), "-A", data, cli, "-t", "i", "-F", str(T1),
+                        "-I", str(T2), "-s", "1"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert open(cli).read() == open(ours).read()
+    exe = os.path.join(ROOT, "oracle", "_ref", "epiGPU-dropin")
+    if os.path.exists(exe):
+        ref = str(tmp_path / "dropin.txt")
+        r = subprocess.run([exe, "-A", data, ref, "-t", "i", "-F", str(T1), "-I", str(T2), "-s", "1"],
+                           capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        assert open(ref).read() == open(ours).read()

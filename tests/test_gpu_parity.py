@@ -190,3 +190,35 @@ def test_hit_overflow_flag(scanner):
     assert full["nhits"] > 1000
     assert small["nhits"] == full["nhits"] and small["overflow"]
     assert len(small["hits"]) == 100
+
+
+@pytest.mark.parametrize("form", ["plain", "mfma"])
+def test_safe_mode_scan(scanner, form):
+    """The reference's safe mode (-s 1): -9 is a phenotype value and test 'i' has no nfac > 6 condition."""
+    nid, sizes = 700, [150, 110]
+    nsnp = sum(sizes)
+    words, y = make_dataset(0x5AFE, nid, nsnp, maf=(0.01, 0.3), gmiss=0.01, pmiss=0.01, sentinel9=40)
+    scanner.set_genotypes(words, nid)
+    scanner.set_phenotypes(y, nine_is_value=True)
+    scanner.set_chromosomes([0, 150], sizes)
+    t1, t2 = 3.0, 4.0
+    print(f"safe scan form={form}", flush=True)
+    g = scanner.scan(test="i", form=form, t1=t1, t2=t2, safe=True)
+    codes = O.unpack(words, nid)
+    n_ref, ref = O.scan(codes, y, [0, 150], sizes, "i", t1, t2, safe=True)
+    n_norm, _ = O.scan(codes, y, [0, 150], sizes, "i", t1, t2)
+    assert n_ref != n_norm  # the two modes differ on this data
+    diff = hit_set_check(g, n_ref, ref, "i", t1, t2)
+    for pr in diff:
+        r = ref[(ref["snp1"] == pr[0]) & (ref["snp2"] == pr[1])]
+        assert len(r) == 0 or min(abs(r["F"][0] - t1) / t1, abs(r["F2"][0] - t2) / t2) < 1e-5, pr
+    rs = {(int(a), int(b)): (f, f2) for a, b, f, f2 in zip(ref["snp1"], ref["snp2"], ref["F"], ref["F2"])}
+    for a, b, f, f2 in zip(g["hits"]["snp1"], g["hits"]["snp2"], g["F64"], g["F2_64"]):
+        if (int(a), int(b)) in rs:
+            assert rel_err(f, rs[(int(a), int(b))][0]) <= TOL_F and rel_err(f2, rs[(int(a), int(b))][1]) <= TOL_F
+    # the per-pair tables with -9 as a value
+    counts, sums, F, F2 = scanner.pair_dump(0, 60, 0, 60, test="i", form=form)
+    sel, orc = oracle_rect(codes, y, 0, 60, 0, 60)
+    orc = O.pairs(codes, y, np.stack(np.nonzero(sel), 1), safe=True)
+    assert (counts[sel] == orc["counts"]).all()
+    scanner.set_phenotypes(y)  # back to normal -9 handling for later tests

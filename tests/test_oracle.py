@@ -143,6 +143,51 @@ def test_oracle_hit_rules():
     assert (r["nfac"] <= 6).any()
 
 
+def test_oracle_safe_mode_nine_is_a_value():
+    """Safe mode (-s 1): -9 is an ordinary phenotype value, so F is the one-way ANOVA over the cells with the
+    -9 individuals included; in normal mode they are excluded. Pinned by scipy's f_oneway on the groups."""
+    from scipy.stats import f_oneway
+    rng = np.random.default_rng(77)
+    for trial in range(20):
+        nid = int(rng.integers(40, 200))
+        ga = rng.integers(0, 3, nid).astype(np.int8)
+        gb = rng.integers(0, 3, nid).astype(np.int8)
+        ga[rng.choice(nid, 3, replace=False)] = 3
+        y = rng.normal(size=nid).astype(np.float32)
+        nine = rng.choice(nid, 6, replace=False)
+        y[nine] = -9.0
+        y[rng.choice(nid, 2, replace=False)] = np.nan
+        codes = np.stack([ga, gb])
+        pr = np.array([[0, 1]], np.int32)
+        for safe in (False, True):
+            r = O.pairs(codes, y, pr, safe=safe)
+            keep = (ga != 3) & (gb != 3) & ~np.isnan(y) & (safe | (y != -9.0))
+            cells = 3 * ga[keep] + gb[keep]
+            groups = [y[keep][cells == c].astype(np.float64) for c in range(9) if (cells == c).any()]
+            assert r["counts"][0].sum() == keep.sum()
+            if len(groups) >= 2 and keep.sum() > len(groups):
+                F = f_oneway(*groups).statistic
+                assert abs(r["F"][0] - F) <= 1e-9 * max(1.0, abs(F))
+
+
+def test_oracle_safe_mode_hit_rule():
+    """Safe test 'i' has no nfac > 6 condition (src/EpiOpenCL.c:949-950); test 'f' is unchanged."""
+    nid = 300
+    words, y = make_dataset(8, nid, 40, maf=(0.02, 0.08))
+    codes = O.unpack(words, nid)
+    aa, bb = np.meshgrid(np.arange(40), np.arange(40), indexing="ij")
+    sel = aa > bb
+    pr = np.stack([aa[sel], bb[sel]], 1)
+    r = O.pairs(codes, y, pr, safe=True)
+    ok = r["df1"] >= 1
+    n_i, _ = O.scan(codes, y, [0], [40], "i", t1=0.5, t2=0.5, safe=True)
+    assert n_i == int((((r["F"] > 0.5) | (r["F2"] > 0.5)) & ok).sum())
+    n_f, _ = O.scan(codes, y, [0], [40], "f", t1=0.5, safe=True)
+    assert n_f == int(((r["F"] > np.float32(0.5)) & ok).sum())
+    n_i0, _ = O.scan(codes, y, [0], [40], "i", t1=0.5, t2=0.5)
+    assert n_i > n_i0  # pairs with nfac <= 6 now count
+
+
 def test_ref32_close_to_oracle():
     """The FP32 reference-shaped baseline tracks the FP64 oracle (its FP32 error is not a parity target)."""
     nid, nsnp = 1000, 60
