@@ -86,7 +86,9 @@ def roofline(form, wl, kernel_ms, traffic=None, source=None):
     if form == "mfma":
         ops, peak, bound, note = 36.0 * wl.nid * wl.pairs, INT8_PEAK_TOPS, "mfma", "INT8 ops; 36*N logical ops per pair"
     else:
-        ops, peak, bound, note = 18.0 * wl.nid * wl.pairs, FP32_PEAK_TFLOPS, "valu", "FP32; 18*N logical flops per pair"
+        ops, peak, bound, note = (18.0 * wl.nid * wl.pairs, FP32_PEAK_TFLOPS, "valu",
+                                  "FP32; 18*N logical flops per pair (9 cell-sum FMAs per individual); FAST tiles "
+                                  "execute 4 of the 9 and derive the rest from per-SNP marginals")
     achieved = ops / (kernel_ms * 1e-3) / 1e12
     return {"bound": bound, "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
             "traffic": traffic, "traffic_source": source, "ops_per_pair": ops / wl.pairs, "note": note}
