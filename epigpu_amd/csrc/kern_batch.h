@@ -5,9 +5,11 @@
 // (tile-major, phenotype-minor) and cuts the list into NQ contiguous queues of
 // equal estimated cost, one per XCD. A workgroup pulls items from the queue of
 // its XCD (workgroups are dealt to XCDs round-robin, so blockIdx % 8 is its
-// XCD) and, when that queue is empty, from the others in turn. Every pull is
-// one atomicAdd; a pull past the end of every queue ends the workgroup, so all
-// workgroups exit once the items run out. The queue heads are reset by a
+// XCD), one item ahead so that the atomic overlaps the current tile, and, when
+// that queue is empty, from the others in turn. Every pull is one atomicAdd; a
+// claimed slot below its queue's end is always processed, and a pull past the
+// end of every queue ends the workgroup, so all workgroups exit once the items
+// run out. The queue heads are reset by a
 // kernel (not a memset) so that the reset and the scan can be captured in one
 // hipGraph and replayed.
 #pragma once
@@ -58,15 +60,27 @@ __global__ __launch_bounds__(fast_threads<TA>(), 64 / TA) void mfma_batch_kernel
     constexpr int LDS = fast_lds<TA>() > general_lds<GA, 1>() ? fast_lds<TA>() : general_lds<GA, 1>();
     __shared__ __attribute__((aligned(16))) uint8_t lds[LDS];
     __shared__ int item;
+    __shared__ int4 wsh;
     const int home = blockIdx.x % NQ;
     const DumpSink nodump = {};
+    // Thread 0 claims the next slot of the home queue one item ahead, so the
+    // atomic's latency overlaps the current tile.
+    int pend = 0;
+    if (threadIdx.x == 0) pend = atomicAdd(&B.qpos[home], 1);
     for (;;) {
-        __syncthreads();  // the previous tile is done with LDS and `item`
-        if (threadIdx.x == 0) item = next_item(B, home);
+        __syncthreads();  // the previous tile is done with LDS and the shared item
+        if (threadIdx.x == 0) {
+            const int it = pend < B.qend[home] ? pend : next_item(B, home);
+            item = it;
+            if (it >= 0) {
+                wsh = B.work[it];
+                if (pend < B.qend[home]) pend = atomicAdd(&B.qpos[home], 1);
+            }
+        }
         __syncthreads();
         const int it = item;
         if (it < 0) break;
-        const int4 w = B.work[it];
+        const int4 w = wsh;
         const int a0 = __builtin_amdgcn_readfirstlane(w.x), b0 = __builtin_amdgcn_readfirstlane(w.y);
         const int ext = __builtin_amdgcn_readfirstlane(w.z), sc = __builtin_amdgcn_readfirstlane(w.w);
         const MfmaScan S = B.mscan[sc];
@@ -85,15 +99,25 @@ template <int PA>
 __global__ __launch_bounds__(256) void plain_batch_kernel(const BatchArgs B)
 {
     __shared__ int item;
+    __shared__ int4 wsh;
     const int home = blockIdx.x % NQ;
     const DumpSink nodump = {};
+    int pend = 0;
+    if (threadIdx.x == 0) pend = atomicAdd(&B.qpos[home], 1);
     for (;;) {
         __syncthreads();
-        if (threadIdx.x == 0) item = next_item(B, home);
+        if (threadIdx.x == 0) {
+            const int it = pend < B.qend[home] ? pend : next_item(B, home);
+            item = it;
+            if (it >= 0) {
+                wsh = B.work[it];
+                if (pend < B.qend[home]) pend = atomicAdd(&B.qpos[home], 1);
+            }
+        }
         __syncthreads();
         const int it = item;
         if (it < 0) break;
-        const int4 w = B.work[it];
+        const int4 w = wsh;
         const int a0 = __builtin_amdgcn_readfirstlane(w.x), b0 = __builtin_amdgcn_readfirstlane(w.y);
         const int ext = __builtin_amdgcn_readfirstlane(w.z), sc = __builtin_amdgcn_readfirstlane(w.w);
         const PlainScan S = B.pscan[sc];
