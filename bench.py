@@ -110,6 +110,31 @@ def accuracy(s, wl, form, phen=0):
             "f2_relerr_max": float(rel(F2.ravel(), ref["F2"]).max())}
 
 
+def run_c2(E, s, steps=3):
+    """c2: N=4000, M=20000 over 2 chromosomes, 1% missing calls, 2% NaN phenotypes, test i: the plain popcount
+    kernel's configuration (every tile is GENERAL), both forms, autotuned schedules, oracle check of each."""
+    from epigpu_amd import workloads
+    wl = workloads.get("c2").generate()
+    s.set_genotypes(wl.words, wl.nid)
+    s.set_phenotypes(wl.y)
+    s.set_chromosomes(wl.chr_starts, wl.chr_sizes)
+    out = {"workload": f"c2: N={wl.nid}, M={wl.nsnp} (2 chr), {wl.pairs} pairs, 1% missing calls, 2% NaN, test i",
+           "pairs_per_step": wl.pairs}
+    for form in ("plain", "mfma"):
+        s.autotune(test=wl.test, form=form, t1=wl.t1, t2=wl.t2, reps=1)
+        kms = []
+        for i in range(steps + 1):
+            r = s.scan(test=wl.test, form=form, t1=wl.t1, t2=wl.t2)
+            if i:
+                kms.append(r["kernel_ms"])
+        km = float(np.mean(kms))
+        out[form] = {"kernel_ms": km, "pairs_per_s": wl.pairs / (km * 1e-3), "hits": r["nhits"],
+                     "schedule": s.schedule(form), "roofline": roofline(form, wl, km), "check": accuracy(s, wl, form)}
+        log(f"[bench c2] {form}: {km:.1f} ms kernel, {out[form]['pairs_per_s']:.3e} pairs/s")
+    out["mfma_over_plain"] = out["plain"]["kernel_ms"] / out["mfma"]["kernel_ms"]
+    return out
+
+
 def run_batch(E, s, steps, warmup, tune=True, plain=True, check=True, sched=None):
     """c4: 64 independent scans of uneven N and M in one captured persistent launch per step."""
     from epigpu_amd import workloads
@@ -368,6 +393,8 @@ def main():
         out["autotune"] = tuned
     if not args.no_batch:
         out["batch_c4"] = run_batch(E, s, args.batch_steps, 1, plain=not args.no_plain, check=not args.no_check)
+        if not args.no_plain and not args.no_check:
+            out["c2"] = run_c2(E, s)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", min(16, os.cpu_count() or 1)))
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds, threads)
