@@ -1,7 +1,7 @@
 """Build recipes for the native libraries, in-tree under epigpu_amd/lib/.
 
   libepiscan.so  hipcc, gfx950 only: HIP kernels + the episcan.h C ABI
-  libepihost.so  gcc: host driver, codec, container, generator, F helpers
+  libepihost.so  gcc: host driver, codec, container, PLINK ingest, generator, F helpers
   episcan        gcc: analysis-mode CLI linked to both
 The oracle (oracle/Makefile) is test infrastructure and is built separately.
 """
@@ -22,7 +22,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-HOST_SRC = ["codec.c", "synth.c", "fdist.c", "container.c", "perform_scan.c", "compat.c"]
+HOST_SRC = ["codec.c", "synth.c", "fdist.c", "container.c", "plink.c", "perform_scan.c", "compat.c"]
 
 
 def _run(cmd, verbose):

@@ -17,6 +17,16 @@
  *                the reference's safe kernels (src/EpiOpenCL.c:505-963)
  *     -i n       accepted for compatibility and ignored (tiles replace launch chunks)
  * Options come in pairs, as in the reference (src/main.c:63).
+ *
+ * Data mode (src/readplink.c:1296-1435):
+ *   episcan -Dr[c][m][n] <file.ped> <file.map> <out.epigpu>   PLINK text to a container
+ *   episcan -Db[c][m][n] <prefix> <out.epigpu>                prefix.bed/.bim/.fam to a container
+ *   episcan -De <file.ped> <file.map> <in.epigpu>             container back to PLINK text
+ *     c  clean: drop low call-rate / low-frequency SNPs, then low call-rate individuals
+ *        and individuals with phenotype -9
+ *     m  impute missing calls from a random individual's call
+ *     n  replace the phenotype with N(0,1) draws
+ * The random steps use the stream the reference's unseeded rand() gives (seed 1).
  */
 #include <ctype.h>
 #include <stdio.h>
@@ -30,7 +40,93 @@ static void usage(void)
 {
     fprintf(stderr,
             "usage: episcan -A <data.epigpu> <output.txt> [-t f|i] [-F x] [-I x] [-f p] [-g p]\n"
-            "               [-d n] [-p n | -p a-b] [-P file] [-m plain|mfma]\n");
+            "               [-d n] [-p n | -p a-b] [-P file] [-m plain|mfma] [-s 0|1]\n"
+            "       episcan -Dr[c][m][n] <file.ped> <file.map> <out.epigpu>\n"
+            "       episcan -Db[c][m][n] <prefix> <out.epigpu>\n"
+            "       episcan -De <file.ped> <file.map> <in.epigpu>\n");
+}
+
+static int ingest_fail(int rc)
+{
+    const char *m = epi_plink_error();
+    fprintf(stderr, "episcan: %s%s%s\n", epih_strerror(rc), m[0] ? ": " : "", m);
+    return 1;
+}
+
+static int datamode(int argc, char **argv)
+{
+    int txt = 0, bed = 0, extract = 0, flags = 0;
+    for (const char *q = argv[1] + 2; *q; q++) {
+        switch (*q) {
+        case 'r': txt = 1; break;
+        case 'b': bed = 1; break;
+        case 'e': extract = 1; break;
+        case 'c': flags |= EPI_PLINK_CLEAN; break;
+        case 'm': flags |= EPI_PLINK_IMPUTE; break;
+        case 'n': flags |= EPI_PLINK_RNORM; break;
+        default:
+            fprintf(stderr, "data mode flag '%c' not supported (r, b, e, c, m, n)\n", *q);
+            return 1;
+        }
+    }
+    if (txt + bed + extract != 1 || (extract && flags) || argc != (bed ? 4 : 5)) {
+        usage();
+        return 1;
+    }
+    int rc;
+    if (extract) {
+        epi_container c;
+        if ((rc = epi_read_container(argv[4], &c))) {
+            fprintf(stderr, "%s: %s\n", argv[4], epih_strerror(rc));
+            return 1;
+        }
+        rc = epi_write_ped_map(&c, argv[2], argv[3]);
+        epi_container_free(&c);
+        if (rc) return ingest_fail(rc);
+        printf("Extracted %s to %s and %s\n", argv[4], argv[2], argv[3]);
+        return 0;
+    }
+    epi_plink d;
+    if (bed) {
+        const char *pre = argv[2];
+        size_t n = strlen(pre) + 5;
+        char *fb = (char *)malloc(n), *fm = (char *)malloc(n), *ff = (char *)malloc(n);
+        if (!fb || !fm || !ff) return 1;
+        snprintf(fb, n, "%s.bed", pre);
+        snprintf(fm, n, "%s.bim", pre);
+        snprintf(ff, n, "%s.fam", pre);
+        rc = epi_read_bed(fb, fm, ff, &d);
+        free(fb);
+        free(fm);
+        free(ff);
+    } else {
+        rc = epi_read_ped_map(argv[2], argv[3], &d);
+    }
+    if (rc) return ingest_fail(rc);
+    printf("%d individuals, %d SNPs, %d chromosomes\n", d.nid, d.nsnp, d.nchr);
+    epi_plink_report rep;
+    if ((rc = epi_plink_process(&d, flags, 1u, &rep))) {
+        epi_plink_free(&d);
+        return ingest_fail(rc);
+    }
+    if (flags & EPI_PLINK_CLEAN)
+        printf("Cleaning kept %d of %d SNPs and %d of %d individuals\n", rep.snps_kept, rep.snps_in, rep.ids_kept,
+               rep.ids_in);
+    if (flags & EPI_PLINK_IMPUTE) printf("%lld missing calls imputed\n", (long long)rep.imputed);
+    if (flags & EPI_PLINK_RNORM) printf("Phenotype replaced by N(0,1) draws\n");
+    epi_container c;
+    rc = epi_plink_to_container(&d, &c);
+    epi_plink_free(&d);
+    if (rc) return ingest_fail(rc);
+    const char *out = argv[argc - 1];
+    rc = epi_write_container(out, &c);
+    epi_container_free(&c);
+    if (rc) {
+        fprintf(stderr, "%s: %s\n", out, epih_strerror(rc));
+        return 1;
+    }
+    printf("Stored as %s\n", out);
+    return 0;
 }
 
 static int all_digits(const char *s)
@@ -63,6 +159,7 @@ static int is_binary(int nid, const ped *dat)
 
 int main(int argc, char **argv)
 {
+    if (argc >= 2 && strncmp(argv[1], "-D", 2) == 0) return datamode(argc, argv);
     if (argc < 4 || strcmp(argv[1], "-A") != 0 || (argc & 1) != 0) {
         usage();
         return 1;

@@ -75,6 +75,20 @@ class ResumeState(ctypes.Structure):
                 ("tothits", ctypes.c_int64), ("blocks", ctypes.c_int)]
 
 
+class Plink(ctypes.Structure):
+    _fields_ = [("nid", ctypes.c_int), ("nsnp", ctypes.c_int), ("nchr", ctypes.c_int), ("dat", ctypes.c_void_p),
+                ("genmap", ctypes.c_void_p), ("chrstat", ctypes.c_void_p), ("codes", ctypes.c_void_p),
+                ("snpchr", ctypes.c_void_p)]
+
+
+class PlinkReport(ctypes.Structure):
+    _fields_ = [("snps_in", ctypes.c_int), ("snps_kept", ctypes.c_int), ("ids_in", ctypes.c_int),
+                ("ids_kept", ctypes.c_int), ("imputed", ctypes.c_int64)]
+
+
+PLINK_CLEAN, PLINK_IMPUTE, PLINK_RNORM = 1, 2, 4
+
+
 class Container(ctypes.Structure):
     _fields_ = [("nid", ctypes.c_int), ("nsnp", ctypes.c_int), ("npack", ctypes.c_int), ("remain", ctypes.c_int),
                 ("nchr", ctypes.c_int), ("dat", ctypes.c_void_p), ("genmap", ctypes.c_void_p),
@@ -173,6 +187,13 @@ def _declare(name, L):
                                                       ctypes.c_int, ctypes.c_char_p, _p(RunReport)]),
             "epi_resume": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, ctypes.c_int, f32, f32, _p(ResumeState)]),
+            "epi_read_ped_map": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _p(Plink)]),
+            "epi_read_bed": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, _p(Plink)]),
+            "epi_plink_process": (ctypes.c_int, [_p(Plink), ctypes.c_int, ctypes.c_uint, _p(PlinkReport)]),
+            "epi_plink_to_container": (ctypes.c_int, [_p(Plink), _p(Container)]),
+            "epi_plink_free": (None, [_p(Plink)]),
+            "epi_write_ped_map": (ctypes.c_int, [_p(Container), ctypes.c_char_p, ctypes.c_char_p]),
+            "epi_plink_error": (ctypes.c_char_p, []),
             "epi_format_row": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
                                               ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p,
                                               ctypes.c_char_p, ctypes.c_int, ctypes.c_int, f32, f32]),
@@ -271,6 +292,83 @@ def read_container(path):
     finally:
         epihost().epi_container_free(ctypes.byref(c))
     return out
+
+
+def _plink_raise(rc):
+    H = epihost()
+    raise EpiError(rc, f"{H.epih_strerror(rc).decode()}: {H.epi_plink_error().decode()}")
+
+
+def _records(ptr, dtype, n):
+    buf = (ctypes.c_char * (dtype.itemsize * n)).from_address(ptr)
+    return np.frombuffer(bytearray(buf), dtype=dtype)
+
+
+def read_plink(files, clean=False, impute=False, rnorm=False, seed=1, container=None):
+    """PLINK data to the scan's inputs (the reference's data mode, src/readplink.c:1296-1435).
+
+    files: (ped, map) for text PLINK, or a fileset prefix string for prefix.bed/.bim/.fam.
+    clean / impute / rnorm: cleangeno, guessmissing and rnorm, in that order, sharing one
+    random stream seeded like srand(seed) (seed 1 = the reference's unseeded run).
+    container: optional path; the packed container is written there.
+    Returns a dict with nid, nsnp, nchr, ped, map, chr records, codes (nsnp x nid uint8,
+    classes 0..3), words (layout A1) and the cleaning report."""
+    H = epihost()
+    d = Plink()
+    if isinstance(files, (str, bytes, os.PathLike)):
+        pre = os.fsdecode(files)
+        rc = H.epi_read_bed(os.fsencode(pre + ".bed"), os.fsencode(pre + ".bim"), os.fsencode(pre + ".fam"),
+                            ctypes.byref(d))
+    else:
+        ped, mp = files
+        rc = H.epi_read_ped_map(os.fsencode(ped), os.fsencode(mp), ctypes.byref(d))
+    if rc:
+        _plink_raise(rc)
+    try:
+        flags = (PLINK_CLEAN if clean else 0) | (PLINK_IMPUTE if impute else 0) | (PLINK_RNORM if rnorm else 0)
+        rep = PlinkReport()
+        rc = H.epi_plink_process(ctypes.byref(d), flags, seed, ctypes.byref(rep))
+        if rc:
+            _plink_raise(rc)
+        c = Container()
+        rc = H.epi_plink_to_container(ctypes.byref(d), ctypes.byref(c))
+        if rc:
+            _plink_raise(rc)
+        try:
+            npack1 = c.npack + (c.remain > 0)
+            out = {
+                "nid": d.nid, "nsnp": d.nsnp, "nchr": d.nchr,
+                "ped": _records(d.dat, PED_DTYPE, d.nid),
+                "map": _records(d.genmap, MAP_DTYPE, d.nsnp),
+                "chr": _records(d.chrstat, CHR_DTYPE, d.nchr),
+                "codes": _records(d.codes, np.dtype(np.uint8), d.nid * d.nsnp).reshape(d.nsnp, d.nid),
+                "words": _records(c.geno, np.dtype("<i4"), npack1 * d.nsnp).reshape(d.nsnp, npack1),
+                "report": {k: getattr(rep, k) for k, _ in PlinkReport._fields_},
+            }
+            if container is not None:
+                rc = H.epi_write_container(os.fsencode(container), ctypes.byref(c))
+                if rc:
+                    raise EpiError(rc, H.epih_strerror(rc).decode())
+        finally:
+            H.epi_container_free(ctypes.byref(c))
+    finally:
+        H.epi_plink_free(ctypes.byref(d))
+    return out
+
+
+def write_ped_map(container, ped, mp):
+    """Container file back to PLINK text (extractfrombinary, src/readplink.c:150-224)."""
+    H = epihost()
+    c = Container()
+    rc = H.epi_read_container(os.fsencode(container), ctypes.byref(c))
+    if rc:
+        raise EpiError(rc, H.epih_strerror(rc).decode())
+    try:
+        rc = H.epi_write_ped_map(ctypes.byref(c), os.fsencode(ped), os.fsencode(mp))
+        if rc:
+            _plink_raise(rc)
+    finally:
+        H.epi_container_free(ctypes.byref(c))
 
 
 def make_records(nid, nsnp, chr_sizes, phen):

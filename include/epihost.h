@@ -136,6 +136,71 @@ void epi_container_free(epi_container *c);
 const char *epih_strerror(int code);
 
 /* ------------------------------------------------------------------------ */
+/* PLINK ingest: the reference's data mode (src/readplink.c:1296-1435).      */
+/*                                                                          */
+/* Genotype classes follow readped (src/readplink.c:234-350): per SNP the    */
+/* allele of the first homozygous call in individual order is class 0, the   */
+/* other homozygote class 2, a heterozygote 1 and a call whose first allele  */
+/* is '0' missing (3). map.allelename holds the alleles as readped leaves    */
+/* them: the last heterozygote's pair in file order, reordered so that the   */
+/* class-0 allele comes first; without heterozygotes {class-0 allele, 0}.    */
+/* Chromosomes are numbered in order of first appearance and must occupy     */
+/* contiguous SNP runs (src/readplink.c:352-415).                            */
+/* Divergences from the reference, all on input it mishandles: tab-separated */
+/* .ped files are read (readped's separator set is " /t"), ragged rows, odd  */
+/* allele counts, names over 29 bytes, non-numeric sex or phenotype,          */
+/* non-contiguous chromosomes and a third homozygous allele are errors, and  */
+/* a SNP without any homozygous call keeps its heterozygote allele names      */
+/* (readped reads past the SNP there).                                      */
+/* ------------------------------------------------------------------------ */
+typedef struct epi_plink {
+    int nid, nsnp, nchr;
+    ped *dat;              /* nid records */
+    map *genmap;           /* nsnp records */
+    chromosome *chrstat;   /* nchr records */
+    uint8_t *codes;        /* nsnp x nid, SNP-major, classes 0..3 */
+    int32_t *snpchr;       /* nsnp chromosome indices */
+} epi_plink;
+
+/* Text .ped (6 columns, then 2 single-character alleles per SNP, space or   */
+/* tab separated) and .map (chr, name, genetic distance, base-pair position).*/
+/* dims/readped/readmap: src/readplink.c:8-60, :234-350, :352-415.           */
+int epi_read_ped_map(const char *pedfile, const char *mapfile, epi_plink *d);
+/* PLINK 1 binary fileset (SNP-major .bed, .bim, .fam): the README's planned */
+/* direct reader (README.md:12-13). Classes and allele names are those       */
+/* epi_read_ped_map gives for the .ped/.map PLINK writes for the same data   */
+/* (heterozygotes as "A1 A2", missing as "0 0"); alleles longer than one     */
+/* character are named by their first character.                            */
+int epi_read_bed(const char *bedfile, const char *bimfile, const char *famfile, epi_plink *d);
+
+/* Data-mode steps, applied in the reference's order (src/readplink.c:1346-1369):
+ * EPI_PLINK_CLEAN   cleangeno (:417-497): keep SNPs with allele-2 frequency >
+ *                   0.01, call rate > 0.9 and position > 0, then individuals
+ *                   with call rate > 0.9 over the kept SNPs and phenotype != -9
+ * EPI_PLINK_IMPUTE  guessmissing (:499-520): every missing call copies the call
+ *                   of a random individual, rand() % nid, drawn until present
+ * EPI_PLINK_RNORM   rnorm (:533-552): phenotypes replaced by Box-Muller draws
+ * The random steps share one stream equal to srand(seed); rand() (the
+ * reference never seeds it in data mode, so seed 1 reproduces its files). */
+#define EPI_PLINK_CLEAN 1
+#define EPI_PLINK_IMPUTE 2
+#define EPI_PLINK_RNORM 4
+typedef struct epi_plink_report {
+    int snps_in, snps_kept;
+    int ids_in, ids_kept;
+    int64_t imputed;
+} epi_plink_report;
+int epi_plink_process(epi_plink *d, int flags, unsigned seed, epi_plink_report *rep);
+/* Packs the codes (layout A1) into a container (pack, src/readplink.c:1066-1102). */
+int epi_plink_to_container(const epi_plink *d, epi_container *c);
+void epi_plink_free(epi_plink *d);
+/* Container back to .ped/.map text (extractfrombinary, src/readplink.c:150-224):
+ * 1/2-coded phenotypes print as integers, others with %f. */
+int epi_write_ped_map(const epi_container *c, const char *pedfile, const char *mapfile);
+/* Message of the last ingest error on this thread ("" if none). */
+const char *epi_plink_error(void);
+
+/* ------------------------------------------------------------------------ */
 /* Scan driver: mirror of perform_scan (src/EpiOpenCL.c:1312-1686).          */
 /* ------------------------------------------------------------------------ */
 typedef struct epi_run_opts {

@@ -8,8 +8,8 @@ cpu_baseline leg may import this package, and only as the checker; the
 product (epigpu_amd/) never does.
 
 Built by oracle/Makefile into oracle/liboracle.so; oracle/_ref/libreadplink.so
-is the reference's own readplink.c compiled from /root/reference (codec and
-container), used to pin the codec.
+is the reference's own readplink.c compiled from /root/reference (codec,
+container and the PLINK data mode), used to pin the codec and the ingest.
 """
 import ctypes
 import os
@@ -115,3 +115,54 @@ def ref32_scan_tiles(words, nid, phen, tiles, tile, test="f", t1=6.5, t2=10.5):
     npairs = lib().ref32_scan_tiles(_p(words), nid, words.shape[0], _p(phen), _p(tl), tl.shape[0], tile, ord(test),
                                     t1, t2, ctypes.byref(nh))
     return npairs, nh.value
+
+
+def ref_data_mode(pedfile, mapfile, out, clean=False, impute=False, rnorm=False):
+    """The reference's data mode -Dr[c][m][n] (src/readplink.c:1330-1435) from its own
+    compiled readplink.c: dims, readped, readmap, cleangeno, guessmissing, rnorm, pack,
+    createpackedbinary into `out`. Record buffers start zero-filled so that padding
+    bytes compare equal, and rand() is seeded with 1 first: the state of the fresh
+    process the reference runs in. Returns (nid, nsnp, nchr) or None when the
+    reference library is not built. Needs nid, nsnp >= 10 (its progress lines
+    divide by n/10)."""
+    L = ref_readplink()
+    if L is None:
+        return None
+    c_int, vp = ctypes.c_int, ctypes.c_void_p
+    nid, nsnp, nchr = c_int(), c_int(), c_int()
+    L.dims(ctypes.byref(nid), ctypes.byref(nsnp), os.fsencode(pedfile))
+    assert nid.value >= 10 and nsnp.value >= 10
+    dat = ctypes.create_string_buffer(128 * nid.value)
+    genmap = ctypes.create_string_buffer(88 * nsnp.value)
+    geno = ctypes.create_string_buffer(nid.value * nsnp.value)
+    chrstat = ctypes.create_string_buffer(44 * 50)
+    L.readped(nid, nsnp, os.fsencode(pedfile), dat, geno, genmap)
+    L.readmap(nsnp, genmap, ctypes.byref(nchr), chrstat, os.fsencode(mapfile))
+    if clean:
+        L.cleangeno(ctypes.byref(nid), ctypes.byref(nsnp), nchr, geno, dat, chrstat, genmap)
+        assert nid.value >= 10 and nsnp.value >= 10
+    libc = ctypes.CDLL(None)
+    libc.srand(1)
+    if impute:
+        L.guessmissing(nid, nsnp, geno)
+    if rnorm:
+        L.rnorm(nid, dat)
+    npack, remain, gpack = c_int(), c_int(), ctypes.POINTER(c_int)()
+    L.pack.argtypes = [c_int, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int),
+                       ctypes.POINTER(ctypes.POINTER(c_int)), vp]
+    L.pack(nid, nsnp, ctypes.byref(npack), ctypes.byref(remain), ctypes.byref(gpack), geno)
+    L.createpackedbinary.argtypes = [c_int, c_int, c_int, c_int, c_int, vp, vp, vp, vp, ctypes.c_char_p]
+    L.createpackedbinary(nid, nsnp, npack, remain, nchr, genmap, dat, ctypes.cast(gpack, vp), chrstat,
+                         os.fsencode(out))
+    libc.free.argtypes = [vp]
+    libc.free(ctypes.cast(gpack, vp))
+    return nid.value, nsnp.value, nchr.value
+
+
+def ref_extract(binfile, pedfile, mapfile):
+    """The reference's -De (extractfrombinary, src/readplink.c:150-224); False when not built."""
+    L = ref_readplink()
+    if L is None:
+        return False
+    L.extractfrombinary(os.fsencode(binfile), os.fsencode(pedfile), os.fsencode(mapfile))
+    return True
