@@ -14,6 +14,7 @@
  *   createpackedbinary / readpacked  src/readplink.c:1145-1273
  *   perform_scan                     src/EpiOpenCL.c:1312-1686, src/epiopencl.h:21
  *   resume                           src/EpiOpenCL.c:1187-1274
+ *   PLINK data mode (ingest, QC)     src/readplink.c:8-560, :1296-1435
  *   analysis-mode defaults           src/main.c:244-249
  */
 #ifndef EPIHOST_H

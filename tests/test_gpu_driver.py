@@ -205,3 +205,31 @@ def test_safe_mode_cli_and_reference_main(tmp_path, dataset):
                            capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
         assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
         assert open(ref).read() == open(ours).read()
+
+
+def test_plink_ingest_then_scan(tmp_path):
+    """PLINK text -> container (-Drcm) -> scan (-A -t i): the hits are the oracle's on the
+    container's genotypes and phenotypes (33 SNPs after cleaning, low thresholds)."""
+    import epigpu_amd as E
+    exe = os.path.join(ROOT, "epigpu_amd", "lib", "episcan")
+    gold = os.path.join(ROOT, "tests", "golden")
+    data = str(tmp_path / "p.epigpu")
+    r = subprocess.run([exe, "-Drcm", os.path.join(gold, "plink_small.ped"), os.path.join(gold, "plink_small.map"),
+                        data], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    c = E.read_container(data)
+    out = str(tmp_path / "out.txt")
+    r = subprocess.run([exe, "-A", data, out, "-t", "i", "-F", "1.5", "-I", "1.5"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    header, blocks, rows = parse_output(out)
+    sizes = c["chr"]["chrsize"].tolist()
+    starts = c["chr"]["chrstart"].tolist()
+    n_ref, ref = O.scan(O.unpack(c["geno"], c["nid"]), c["ped"]["phen"], starts, sizes, "i", 1.5, 1.5)
+    got = {(a, b) for rs in rows.values() for a, b, *_ in rs}
+    want = set(zip(ref["snp1"].tolist(), ref["snp2"].tolist()))
+    assert n_ref > 10
+    for pr in got ^ want:  # only threshold ties may differ
+        x = ref[(ref["snp1"] == pr[0]) & (ref["snp2"] == pr[1])]
+        assert len(x) == 0 or min(abs(x["F"][0] - 1.5), abs(x["F2"][0] - 1.5)) < 1e-5 * 1.5
+    assert len(got ^ want) <= 2
