@@ -72,7 +72,7 @@ struct MfmaArgs {
     DumpSink ds;
 };
 
-constexpr int FAST_LDS = 2 * (2 * (64 + 64) + 4) * ROWB;     // largest FAST tile, two stage buffers
+constexpr int FAST_LDS = 2 * (2 * (64 + 64) + 5) * ROWB;     // largest FAST tile, two stage buffers
 
 // blockIdx -> tile: consecutive tiles (which share SNP data) go to the same
 // XCD. Workgroups are dealt to the 8 XCDs round-robin, so within each full
@@ -120,15 +120,25 @@ __device__ __forceinline__ v4i expand16_sel(uint32_t x, bool ff)
 // plane, becomes 32 bytes of one LDS row), and the NWR weight rows in 8-byte
 // pieces by the first NWR / 4 waves. Two register sets hold the stages in
 // flight, so that a stage's loads are issued two stages before it is written
-// to LDS. A rows are expanded to 0xFF bytes, B rows to 0x01 bytes.
+// to LDS. A rows are expanded to 0xFF bytes, B rows to 0x01 bytes. Loads and
+// stores are branch-free within a wave, so that a stage is straight-line code
+// the scheduler can interleave with MFMAs: waves that load no weights write
+// their (in-range, unused) weight load to a dummy LDS row that no operand
+// fetch reads. When the dwords do not fill the last round of threads, the
+// idle lanes form whole waves, which skip that round with a wave-uniform
+// branch; the lanes' dummy destination covers any other case.
 template <int NPL, int TA, int TB, int NB, int NWR, int NT>
 struct Stager {
     static constexpr int NWD = KC / 32;                 // words per row per stage
     static constexpr int NA = NPL * TA * NWD;           // A-row dwords per stage
     static constexpr int NDW = NPL * (TA + TB) * NWD;   // dwords per stage
     static constexpr int ROWS = NPL * (TA + TB);        // weight rows follow the SNP rows
+    static constexpr int DUMMY = ROWS + NWR;            // written by idle lanes, never read
     static_assert(NB * NT >= NDW && (NB - 1) * NT < NDW, "dwords per thread");
     static_assert(NWR % 4 == 0 && NWR * KC / 8 == 64 * (NWR / 4), "weight rows per wave");
+    static constexpr bool PART = NB * NT > NDW;         // the last dword round leaves lanes idle
+    static_assert((NDW - (NB - 1) * NT) % 64 == 0, "idle lanes fill whole waves");
+    bool pact;                                          // this wave has a dword in the last round
     const uint32_t *src[NB];
     int dst[NB];
     bool act[NB], isa[NB];
@@ -156,12 +166,13 @@ struct Stager {
             snp = snp < nsnp ? snp : nsnp - 1;
             src[i] = bits + ((size_t)plane * Kw + wd) * nsnp + snp;
             const int row = isa[i] ? plane * TA + sl : NPL * TA + plane * TB + sl;
-            dst[i] = row * ROWB + wd * 32;
+            dst[i] = act[i] ? row * ROWB + wd * 32 : DUMMY * ROWB;
         }
+        pact = (__builtin_amdgcn_readfirstlane(tid) & ~63) + (NB - 1) * NT < NDW;
         const int wv = tid >> 6, r = tid >> 4, ch = tid & 15;
         wload = wv < NWR / 4;
         wsrc = wts + (size_t)(wload ? r : 0) * Kp + ch * 8;
-        wdst = (ROWS + r) * ROWB + ch * 8;
+        wdst = wload ? (ROWS + r) * ROWB + ch * 8 : DUMMY * ROWB + 32 + 8 * (tid & 7);
         w[0] = w[1] = make_int2(0, 0);
     }
     // stage starting at individual k0 (a multiple of KC) into register set S
@@ -171,15 +182,15 @@ struct Stager {
         const size_t off = (size_t)(k0 / 32) * nsnp;
 #pragma unroll
         for (int i = 0; i < NB; i++)
-            if (NT * (i + 1) <= NDW || act[i]) v[S][i] = src[i][off];
-        if (wload) w[S] = *reinterpret_cast<const int2 *>(wsrc + k0);
+            if (!PART || i < NB - 1 || pact) v[S][i] = src[i][off];
+        w[S] = *reinterpret_cast<const int2 *>(wsrc + k0);
     }
     template <int S>
     __device__ __forceinline__ void store(uint8_t *st) const
     {
 #pragma unroll
         for (int i = 0; i < NB; i++) {
-            if (!(NT * (i + 1) <= NDW || act[i])) continue;
+            if (PART && i == NB - 1 && !pact) continue;
             v4i lo, hi;
             if (NT * (i + 1) <= NA) {
                 lo = expand16<true>(v[S][i] & 0xFFFFu);
@@ -194,7 +205,7 @@ struct Stager {
             *reinterpret_cast<v4i *>(st + dst[i]) = lo;
             *reinterpret_cast<v4i *>(st + dst[i] + 16) = hi;
         }
-        if (wload) *reinterpret_cast<int2 *>(st + wdst) = w[S];
+        *reinterpret_cast<int2 *>(st + wdst) = w[S];
     }
 };
 
@@ -207,39 +218,55 @@ struct SetTag {
 // of KC through two LDS buffers. Within a stage the operands of the next
 // 32-individual step are read from LDS (fetch) while the MFMAs of the
 // current one run (issue); the next stage is written to the other buffer
-// mid-stage and one barrier per stage publishes it.
-template <class St, class Ops, class Fetch, class Issue>
+// mid-stage and one barrier per stage publishes it. NMF is the number of
+// MFMAs per step (per issue).
+template <class St, class Ops, int NMF, class Fetch, class Issue>
 __device__ __forceinline__ void k_loop(St &sg, uint8_t *lds, int stage_bytes, int Kp, Fetch fetch, Issue issue)
 {
-    Ops o0, o1;
+    Ops o0, o1 = {};
     sg.template load<0>(0);
     sg.template load<1>(KC);
     sg.template store<0>(lds);
-    if (2 * KC < Kp) sg.template load<0>(2 * KC);
+    sg.template load<0>(2 * KC < Kp ? 2 * KC : 0);
     __syncthreads();
-    fetch(lds, 0, o0);
+    // One stage per region between barriers: step 3 of the previous stage
+    // (o1; zero operands before the first stage) and steps 0-2 of this one
+    // are issued while this stage's operands are read from LDS and the next
+    // stage is expanded into the other buffer.
     auto body = [&](auto tag, const uint8_t *cur, uint8_t *nxt, int kc) {
         constexpr int S = decltype(tag)::v;
-        const bool more = kc + KC < Kp;
+        fetch(cur, 0, o0);
+        issue(o1);
         fetch(cur, 1, o1);
         issue(o0);
         fetch(cur, 2, o0);
-        if (more) {
-            sg.template store<S>(nxt);
-            if (kc + 3 * KC < Kp) sg.template load<S>(kc + 3 * KC);
-        }
+        sg.template store<S>(nxt);
+        sg.template load<S>(kc + 3 * KC < Kp ? kc + 3 * KC : Kp - KC);
         issue(o1);
         fetch(cur, 3, o1);
         issue(o0);
+        // Instruction order of the stage: after each MFMA up to five VALU
+        // instructions (expansion, AND) and one LDS read, an LDS write every
+        // fourth MFMA and the global loads after the fourth, so that the
+        // MFMA pipe stays fed while the wave does its share of staging.
+        // Masks: 0x008 MFMA, 0x002 VALU, 0x100 LDS read, 0x200 LDS write,
+        // 0x020 global read.
+#pragma unroll
+        for (int u = 0; u < 4 * NMF; u++) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+            if (u % 4 == 2) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+            if (u >= 4 && u < 8) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+        }
         // the next stage was written by every wave during this one
         __syncthreads();
-        if (more) fetch(nxt, 0, o0);
-        issue(o1);
     };
     for (int k0 = 0; k0 < Kp; k0 += 2 * KC) {
         body(SetTag<1>{}, lds, lds + stage_bytes, k0);
         body(SetTag<0>{}, lds + stage_bytes, lds, k0 + KC);
     }
+    issue(o1);
 }
 
 // ---------------------------------------------------------------- FAST ----
@@ -248,7 +275,7 @@ __device__ __forceinline__ void k_loop(St &sg, uint8_t *lds, int stage_bytes, in
 // 32-individual step and 128 accumulator registers per lane, so a SIMD runs
 // two waves (two 256-thread workgroups, or one of 512 threads, per CU).
 // LDS rows per stage: A [plane 0..1][TA a], B [plane 0..1][64 b],
-// W [V, L0, L1, L2]. Two stage buffers; operands of the next 32-individual
+// W [V, L0, L1, L2], a dummy row. Two stage buffers; operands of the next 32-individual
 // step are read from LDS while the MFMAs of the current step run.
 template <int TA>
 constexpr int fast_threads() { return 8 * TA; }
@@ -263,7 +290,7 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
     constexpr int APW = 8;                    // a-SNPs per wave
     constexpr int ROWS_AB = NPL * (TA + TB);  // 192 or 256
     constexpr int NCH = ROWS_AB * (KC / 32) / NT;  // bitplane dwords per thread per stage
-    constexpr int STAGE = (ROWS_AB + 4) * ROWB;
+    constexpr int STAGE = (ROWS_AB + 5) * ROWB;
     static_assert(2 * STAGE <= FAST_LDS, "LDS size");
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -312,7 +339,7 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
         }
     };
 
-    k_loop<decltype(sg), Ops>(sg, lds, STAGE, Kp, fetch, issue);
+    k_loop<decltype(sg), Ops, 2 * 4>(sg, lds, STAGE, Kp, fetch, issue);
 
     // Per-SNP class marginals of the tile, staged in LDS (the operand buffers
     // are free once every wave has left the K loop).
@@ -394,7 +421,7 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
 }
 
 template <int TA>
-constexpr int fast_lds() { return 2 * (2 * (TA + 64) + 4) * ROWB; }
+constexpr int fast_lds() { return 2 * (2 * (TA + 64) + 5) * ROWB; }
 
 template <int TA, bool DUMP>
 __global__ __launch_bounds__(fast_threads<TA>(), 64 / TA) void mfma_fast_kernel(const MfmaArgs A)
@@ -413,7 +440,7 @@ __global__ __launch_bounds__(fast_threads<TA>(), 64 / TA) void mfma_fast_kernel(
 // zero row. Lane half hf holds a-SNP hf. NBG = 1 holds 96 accumulator
 // registers per lane (two waves per SIMD), NBG = 2 holds 192 (one).
 template <int GA, int NBG>
-constexpr int general_lds() { return 2 * (3 * (GA + 32 * NBG) + NWP) * ROWB; }
+constexpr int general_lds() { return 2 * (3 * (GA + 32 * NBG) + NWP + 1) * ROWB; }
 
 template <int GA>
 constexpr int general_threads() { return 16 * GA; }
@@ -428,7 +455,7 @@ __device__ __forceinline__ void mfma_general_tile(const MfmaScan &A, const HitSi
     constexpr int TA = GA, TB = 32 * NBG, NPL = 3, NT = general_threads<GA>();
     constexpr int ROWS_AB = NPL * (TA + TB);
     constexpr int NCH = (NPL * (TA + TB) * (KC / 32) + NT - 1) / NT;  // bitplane dwords per thread
-    constexpr int STAGE = (ROWS_AB + NW + 1) * ROWB;    // + one zero row
+    constexpr int STAGE = (ROWS_AB + NW + 2) * ROWB;    // + one zero row, one dummy row
     static_assert(2 * STAGE == general_lds<GA, NBG>(), "LDS size");
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -476,7 +503,7 @@ __device__ __forceinline__ void mfma_general_tile(const MfmaScan &A, const HitSi
         }
     };
 
-    k_loop<decltype(sg), Ops>(sg, lds, STAGE, Kp, fetch, issue);
+    k_loop<decltype(sg), Ops, 2 * 3 * NBG>(sg, lds, STAGE, Kp, fetch, issue);
 
     // Epilogue: lane holds pairs (a = a0 + 4w + 2i + hf, b = b0 + 32bg + m);
     // acc[i][3bg+h][4g + w] = X[g][h][w], g = 3 -> Z weights on NM.
