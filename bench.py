@@ -64,16 +64,19 @@ def cpu_baseline(wl, seconds, threads):
 # schedule (FETCH_SIZE + WRITE_SIZE, each scaled by its factor on a copy of known size).
 # Profiled at FAST tile height 32 with 16 x 2 super-tiles; other super-tile shapes of the same tile height
 # differ only in L2 reuse, and HBM runs at about 1% of its bandwidth either way.
-PROFILED = {("c3", "mfma", 32): "profiles/mfma_fast_kernel_c3.md"}
+# rocprofv3 summaries holding the HBM bytes per launch of the headline kernel, per FAST tile height
+PROFILED = {("c3", "mfma", 64): ("profiles/mfma_fast_kernel_c3.md", "TRAFFIC_BYTES_PER_LAUNCH"),
+            ("c3", "mfma", 32): ("profiles/mfma_fast_kernel_c3.md", "TRAFFIC_BYTES_PER_LAUNCH_32")}
 
 
 def profiled_traffic(workload, form, sched):
-    path = PROFILED.get((workload, form, sched["fast_rows"]))
+    path, key = PROFILED.get((workload, form, sched["fast_rows"]), (None, None))
     if not path or not os.path.exists(os.path.join(ROOT, path)):
         return None
     for line in open(os.path.join(ROOT, path)):
-        if line.startswith("TRAFFIC_BYTES_PER_LAUNCH"):
-            return float(line.split()[1])
+        f = line.split()
+        if len(f) == 2 and f[0] == key:
+            return float(f[1])
     return None
 
 
@@ -371,7 +374,7 @@ def main():
                    "form": args.form, "pairs_per_step": wl.pairs},
         "kernel_ms_per_step": kernel_ms,
         "roofline": roofline(args.form, wl, kernel_ms, traffic,
-                             PROFILED.get((wl.name, args.form, s.schedule(args.form)["fast_rows"]))),
+                             PROFILED.get((wl.name, args.form, s.schedule(args.form)["fast_rows"]), (None,))[0]),
         "hits": int(r["nhits"]), "planted_found": found,
         "prep_ms": prep_s * 1e3,
     }
