@@ -421,12 +421,12 @@ PlainScan plain_scan(const Dataset &d, int r, int tag)
 }
 
 // Schedules. FAST tiles (no missing call in either SNP range) are
-// fast_rows x 64. GENERAL tiles are fast_rows/2 x 32 in the MFMA form (the
+// fast_rows x 64. GENERAL tiles are fast_rows x 32 in the MFMA form (the
 // same workgroup size as its FAST tiles, two waves per SIMD) and 16 x 64 in
 // the plain form.
 constexpr int TILE_B = 64;
 
-int gen_rows(int form, const epi_schedule &s) { return form == EPI_FORM_MFMA ? s.fast_rows / 2 : 16; }
+int gen_rows(int form, const epi_schedule &s) { return form == EPI_FORM_MFMA ? s.fast_rows : 16; }
 int gen_cols(int form, const epi_schedule &) { return form == EPI_FORM_MFMA ? 32 : 64; }
 
 epi_schedule default_schedule(int form)
@@ -588,10 +588,10 @@ int launch_pairs(epi_ctx *c, const epi_scan_params *p, const epi_schedule &sc, c
             A.tiles = c->tiles + nf;
             A.ntiles = ng;
             if (sc.fast_rows == 64)
-                hipLaunchKernelGGL((mfma_general_kernel<32, 1, DUMP>), dim3(ng), dim3(general_threads<32>()), 0,
+                hipLaunchKernelGGL((mfma_general_kernel<64, DUMP>), dim3(ng), dim3(general_threads<64>()), 0,
                                    c->stream, A);
             else
-                hipLaunchKernelGGL((mfma_general_kernel<16, 1, DUMP>), dim3(ng), dim3(general_threads<16>()), 0,
+                hipLaunchKernelGGL((mfma_general_kernel<32, DUMP>), dim3(ng), dim3(general_threads<32>()), 0,
                                    c->stream, A);
             HIPCHK(c, hipGetLastError());
             c->last_launches++;
@@ -677,7 +677,7 @@ int build_batch_work(epi_batch *b, int form, const epi_schedule &sched)
         ct[kind].resize(nd);
     }
     b->pairs = 0;
-    const double unit_fast = form == EPI_FORM_MFMA ? 16.0 : 4.0, unit_gen = form == EPI_FORM_MFMA ? 48.0 : 10.0;
+    const double unit_fast = form == EPI_FORM_MFMA ? 16.0 : 4.0, unit_gen = form == EPI_FORM_MFMA ? 40.0 : 10.0;
     for (int di = 0; di < nd; di++) {
         const Dataset &d = *b->data[di];
         std::vector<int> sc;

@@ -50,14 +50,14 @@ __device__ __forceinline__ int next_item(const BatchArgs &B, int home)
     return -1;
 }
 
-// TA: FAST tiles are TA x 64 (TA = 32 or 64), GENERAL tiles TA/2 x 32; a
+// TA: FAST tiles are TA x 64 (TA = 32 or 64), GENERAL tiles TA x 32; a
 // workgroup has 8 TA threads for both.
 template <int TA>
 __global__ __launch_bounds__(fast_threads<TA>(), 64 / TA) void mfma_batch_kernel(const BatchArgs B)
 {
-    constexpr int GA = TA / 2;
+    constexpr int GA = TA;
     static_assert(general_threads<GA>() == fast_threads<TA>(), "one workgroup size for both tile kinds");
-    constexpr int LDS = fast_lds<TA>() > general_lds<GA, 1>() ? fast_lds<TA>() : general_lds<GA, 1>();
+    constexpr int LDS = fast_lds<TA>() > general_lds<GA>() ? fast_lds<TA>() : general_lds<GA>();
     __shared__ __attribute__((aligned(16))) uint8_t lds[LDS];
     __shared__ int item;
     __shared__ int4 wsh;
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(fast_threads<TA>(), 64 / TA) void mfma_batch_kernel
         if ((ext >> 16) == 0)
             mfma_fast_tile<TA, false>(S, hs, nodump, a0, a1, b0, b1, lds);
         else
-            mfma_general_tile<GA, 1, false>(S, hs, nodump, a0, a1, b0, b1, lds);
+            mfma_general_tile<GA, false>(S, hs, nodump, a0, a1, b0, b1, lds);
     }
 }
 

@@ -269,6 +269,50 @@ __device__ __forceinline__ void k_loop(St &sg, uint8_t *lds, int stage_bytes, in
     issue(o1);
 }
 
+// The same K loop with one operand set: every step issues the MFMAs of the
+// operands in registers and then reads the next step's operands into the
+// same registers, so that a register is refilled as soon as its last MFMA or
+// AND has read it (the issue order decides how early). For tiles whose
+// accumulators leave no room for a second operand set.
+template <class St, class Ops, int NMF, class Fetch, class Issue>
+__device__ __forceinline__ void k_loop1(St &sg, uint8_t *lds, int stage_bytes, int Kp, Fetch fetch, Issue issue)
+{
+    Ops o = {};
+    sg.template load<0>(0);
+    sg.template load<1>(KC);
+    sg.template store<0>(lds);
+    sg.template load<0>(2 * KC < Kp ? 2 * KC : 0);
+    __syncthreads();
+    auto body = [&](auto tag, const uint8_t *cur, uint8_t *nxt, int kc) {
+        constexpr int S = decltype(tag)::v;
+        issue(o);  // step 3 of the previous stage (zero operands before the first)
+        fetch(cur, 0, o);
+        issue(o);
+        fetch(cur, 1, o);
+        issue(o);
+        fetch(cur, 2, o);
+        sg.template store<S>(nxt);
+        sg.template load<S>(kc + 3 * KC < Kp ? kc + 3 * KC : Kp - KC);
+        issue(o);
+        fetch(cur, 3, o);
+#pragma unroll
+        for (int u = 0; u < 4 * NMF; u++) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+            if (u % 4 == 2) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+            if (u >= 4 && u < 8) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+        }
+        // the next stage was written by every wave during this one
+        __syncthreads();
+    };
+    for (int k0 = 0; k0 < Kp; k0 += 2 * KC) {
+        body(SetTag<1>{}, lds, lds + stage_bytes, k0);
+        body(SetTag<0>{}, lds + stage_bytes, lds, k0 + KC);
+    }
+    issue(o);
+}
+
 // ---------------------------------------------------------------- FAST ----
 // Tile TA a x 64 b (TA = 32 or 64) with TA / 8 waves; wave w owns a-SNPs
 // [8w, 8w+8) x all 64 b as two blocks of 32 A rows: 8 MFMAs per
@@ -434,128 +478,132 @@ __global__ __launch_bounds__(fast_threads<TA>(), 64 / TA) void mfma_fast_kernel(
 }
 
 // ------------------------------------------------------------- GENERAL ----
-// Tile GA a x TB b (GA = 16 or 32, TB = 32 NBG = 32 or 64) with GA / 4
-// waves; wave w owns a-SNPs [4w, 4w+4) x all TB b. A operand: 2 a-SNPs x 16
-// rows: (g in {0,1,NM}) x (V, L0, L1, L2), then (NM) x (Z0, Z1, Z2), then a
-// zero row. Lane half hf holds a-SNP hf. NBG = 1 holds 96 accumulator
-// registers per lane (two waves per SIMD), NBG = 2 holds 192 (one).
-template <int GA, int NBG>
-constexpr int general_lds() { return 2 * (3 * (GA + 32 * NBG) + NWP + 1) * ROWB; }
+// Tile GA a x 32 b (GA = 32 or 64) with GA / 8 waves; wave w owns a-SNPs
+// [8w, 8w+8) x the 32 b. Three A blocks, one per class g of the a-SNP
+// (0, 1, NM), of 32 rows each: row m is a-SNP 4((m>>2)&1) + (m>>3) of the
+// wave's eight with weight m & 3 (V, L0, L1, L2), so that lane half hf of the
+// 32x32 accumulator holds a-SNPs 4hf..4hf+3 with all four weights. Each A
+// block meets the three B classes (0, 1, NM): 9 MFMAs. A fourth block, the
+// same NM masks with weights Z0, Z1, Z2 and the zero row, meets only the NM
+// class of B for the complete-case sum of q^2: 10 MFMAs per 32-individual
+// step for 8 x 32 pairs (40 MACs per pair and individual, 39 of them needed)
+// and 160 accumulator registers, two waves per SIMD.
+template <int GA>
+constexpr int general_lds() { return 2 * (3 * (GA + 32) + NWP + 1) * ROWB; }
 
 template <int GA>
-constexpr int general_threads() { return 16 * GA; }
+constexpr int general_threads() { return 8 * GA; }
 
-template <int GA, int NBG, bool DUMP>
+template <int GA, bool DUMP>
 __device__ __forceinline__ void mfma_general_tile(const MfmaScan &A, const HitSink &hs, const DumpSink &dsk, int a0,
                                                   int a1, int b0, int b1, uint8_t *lds)
 {
-    static_assert(NBG == 1 || NBG == 2, "general tile width");
-    static_assert(GA == 16 || GA == 32, "general tile height");
-    static_assert(GA == 16 || NBG == 1, "eight waves of 192 accumulators do not fit a CU");
-    constexpr int TA = GA, TB = 32 * NBG, NPL = 3, NT = general_threads<GA>();
+    static_assert(GA == 32 || GA == 64, "general tile height");
+    constexpr int TA = GA, TB = 32, NPL = 3, NT = general_threads<GA>();
     constexpr int ROWS_AB = NPL * (TA + TB);
-    constexpr int NCH = (NPL * (TA + TB) * (KC / 32) + NT - 1) / NT;  // bitplane dwords per thread
-    constexpr int STAGE = (ROWS_AB + NW + 2) * ROWB;    // + one zero row, one dummy row
-    static_assert(2 * STAGE == general_lds<GA, NBG>(), "LDS size");
+    constexpr int NCH = (ROWS_AB * (KC / 32) + NT - 1) / NT;  // bitplane dwords per thread
+    constexpr int STAGE = (ROWS_AB + NWP + 1) * ROWB;          // + weight rows (the last all zero), a dummy row
+    static_assert(2 * STAGE == general_lds<GA>(), "LDS size");
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int Kp = A.Kp;
 
-    // Global -> LDS: ROWS_AB rows of bitplanes, expanded to bytes, and the NW
-    // weight rows plus the all-zero row NW that the padding A row reads.
+    // Global -> LDS: ROWS_AB rows of bitplanes, expanded to bytes, and the NWP
+    // weight rows V, L0, L1, L2, Z0, Z1, Z2 and the all-zero row.
     Stager<NPL, TA, TB, NCH, NWP, NT> sg(A.bits, A.wts, Kp, A.nsnp, a0, b0, tid);
 
     const int m = lane & 31, hf = lane >> 5;
-    const int aj = (m >> 2) & 1;          // a-SNP within the operand
-    const int q = m >> 3, r4 = m & 3;     // row group and item
-    const int aplane = q < 3 ? q : 2;
-    const int wrow = q < 3 ? r4 : (r4 < 3 ? 4 + r4 : NW);  // NW = zero row
-    const int offW = (ROWS_AB + wrow) * ROWB + 16 * hf;
-    const int offB = (NPL * TA + m) * ROWB + 16 * hf;                  // + (h*TB + 32*bg) * ROWB
-    const int offA = (aplane * TA + 4 * wid + aj) * ROWB + 16 * hf;    // + 2*i*ROWB
+    const int aj = 4 * ((m >> 2) & 1) + (m >> 3);  // a-SNP of A row m among the wave's eight
+    const int wr = m & 3;                          // weight of A row m
+    const int offA = (8 * wid + aj) * ROWB + 16 * hf;                           // + g * TA * ROWB
+    const int offW = (ROWS_AB + wr) * ROWB + 16 * hf;                           // V, L0, L1, L2
+    const int offZ = (ROWS_AB + (wr < 3 ? 4 + wr : NW)) * ROWB + 16 * hf;       // Z0, Z1, Z2, zero row
+    const int offB = (NPL * TA + m) * ROWB + 16 * hf;                           // + h * TB * ROWB
 
-    v16i acc[2][3 * NBG];
+    v16i acc[3][3], accz = (v16i){};
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int g = 0; g < 3; g++)
 #pragma unroll
-        for (int j = 0; j < 3 * NBG; j++) acc[i][j] = (v16i){};
+        for (int h = 0; h < 3; h++) acc[g][h] = (v16i){};
 
     struct Ops {
-        v4i w, b[3 * NBG], am[2];
+        v4i w, z, b[3], am[3];
     };
     auto fetch = [&](const uint8_t *st, int t, Ops &o) {
         const int k = 32 * t;
         o.w = lds_v4(st + offW + k);
+        o.z = lds_v4(st + offZ + k);
 #pragma unroll
-        for (int bg = 0; bg < NBG; bg++)
+        for (int h = 0; h < 3; h++) o.b[h] = lds_v4(st + offB + h * TB * ROWB + k);
 #pragma unroll
-            for (int h = 0; h < 3; h++) o.b[3 * bg + h] = lds_v4(st + offB + (h * TB + 32 * bg) * ROWB + k);
-#pragma unroll
-        for (int i = 0; i < 2; i++) o.am[i] = lds_v4(st + offA + 2 * i * ROWB + k);
+        for (int g = 0; g < 3; g++) o.am[g] = lds_v4(st + offA + g * TA * ROWB + k);
     };
+    // B class outermost: b[0] and b[1] are free for the next step's reads
+    // after the third and sixth MFMA, the A masks and weights after the ANDs.
     auto issue = [&](const Ops &o) {
+        v4i Aop[3];
 #pragma unroll
-        for (int i = 0; i < 2; i++) {
-            const v4i Aop = o.am[i] & o.w;
+        for (int g = 0; g < 3; g++) Aop[g] = o.am[g] & o.w;
+        const v4i Az = o.am[2] & o.z;
 #pragma unroll
-            for (int j = 0; j < 3 * NBG; j++)
-                acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(Aop, o.b[j], acc[i][j], 0, 0, 0);
-        }
+        for (int h = 0; h < 3; h++)
+#pragma unroll
+            for (int g = 0; g < 3; g++)
+                acc[g][h] = __builtin_amdgcn_mfma_i32_32x32x32_i8(Aop[g], o.b[h], acc[g][h], 0, 0, 0);
+        accz = __builtin_amdgcn_mfma_i32_32x32x32_i8(Az, o.b[2], accz, 0, 0, 0);
     };
 
-    k_loop<decltype(sg), Ops, 2 * 3 * NBG>(sg, lds, STAGE, Kp, fetch, issue);
+    k_loop1<decltype(sg), Ops, 10>(sg, lds, STAGE, Kp, fetch, issue);
 
-    // Epilogue: lane holds pairs (a = a0 + 4w + 2i + hf, b = b0 + 32bg + m);
-    // acc[i][3bg+h][4g + w] = X[g][h][w], g = 3 -> Z weights on NM.
+    // Epilogue: lane holds pairs (a = a0 + 8w + 4hf + j, b = b0 + m), j = 0..3;
+    // acc[g][h][4j + w] = X[g][h][w], accz[4j + w] = X[NM][NM][Z_w].
+    const int b = b0 + m;
 #pragma unroll
-    for (int i = 0; i < 2; i++) {
-        const int a = a0 + 4 * wid + 2 * i + hf;
+    for (int j = 0; j < 4; j++) {
+        const int a = a0 + 8 * wid + 4 * hf + j;
+        int xc[3][3];
+        float xf[3][3];
 #pragma unroll
-        for (int bg = 0; bg < NBG; bg++) {
-            const int b = b0 + 32 * bg + m;
-            int xc[3][3];
-            float xf[3][3];
+        for (int g = 0; g < 3; g++)
+#pragma unroll
+            for (int h = 0; h < 3; h++) {
+                const v16i &v = acc[g][h];
+                xc[g][h] = v[4 * j];
+                xf[g][h] = (float)v[4 * j + 1] + 256.0f * (float)v[4 * j + 2] + 65536.0f * (float)v[4 * j + 3];
+            }
+        const float sqf =
+            ((float)accz[4 * j] + 256.0f * (float)accz[4 * j + 1] + 65536.0f * (float)accz[4 * j + 2]) * 8388608.0f;
+        int n[9];
+        float sf[9];
+        cells_from_x(xc, n);
+        cells_from_x(xf, sf);
+        // integer-valued doubles below 2^53: exact
+        auto exact = [&](double sd[9], double &sq) {
+            double xs[3][3];
 #pragma unroll
             for (int g = 0; g < 3; g++)
 #pragma unroll
                 for (int h = 0; h < 3; h++) {
-                    const v16i &v = acc[i][3 * bg + h];
-                    xc[g][h] = v[4 * g];
-                    xf[g][h] = (float)v[4 * g + 1] + 256.0f * (float)v[4 * g + 2] + 65536.0f * (float)v[4 * g + 3];
+                    const v16i &v = acc[g][h];
+                    xs[g][h] = (double)v[4 * j + 1] + 256.0 * (double)v[4 * j + 2] + 65536.0 * (double)v[4 * j + 3];
                 }
-            const v16i &vz = acc[i][3 * bg + 2];
-            const float sqf = ((float)vz[12] + 256.0f * (float)vz[13] + 65536.0f * (float)vz[14]) * 8388608.0f;
-            int n[9];
-            float sf[9];
-            cells_from_x(xc, n);
-            cells_from_x(xf, sf);
-            // integer-valued doubles below 2^53: exact
-            auto exact = [&](double sd[9], double &sq) {
-                double xs[3][3];
-#pragma unroll
-                for (int g = 0; g < 3; g++)
-#pragma unroll
-                    for (int h = 0; h < 3; h++) {
-                        const v16i &v = acc[i][3 * bg + h];
-                        xs[g][h] = (double)v[4 * g + 1] + 256.0 * (double)v[4 * g + 2] + 65536.0 * (double)v[4 * g + 3];
-                    }
-                cells_from_x(xs, sd);
-                sq = ((double)vz[12] + 256.0 * (double)vz[13] + 65536.0 * (double)vz[14]) * 8388608.0;
-            };
-            const bool valid = a < a1 && b < b1 && a > b;
-            finish_pair<DUMP>(hs, dsk, valid, a, b, n, sf, sqf, exact, CLS_IDENTITY, CLS_IDENTITY);
-        }
+            cells_from_x(xs, sd);
+            sq = ((double)accz[4 * j] + 256.0 * (double)accz[4 * j + 1] + 65536.0 * (double)accz[4 * j + 2]) *
+                 8388608.0;
+        };
+        const bool valid = a < a1 && b < b1 && a > b;
+        finish_pair<DUMP>(hs, dsk, valid, a, b, n, sf, sqf, exact, CLS_IDENTITY, CLS_IDENTITY);
     }
 }
 
-template <int GA, int NBG, bool DUMP>
-__global__ __launch_bounds__(general_threads<GA>(), (3 - NBG) * 16 / GA) void mfma_general_kernel(const MfmaArgs A)
+template <int GA, bool DUMP>
+__global__ __launch_bounds__(general_threads<GA>(), 64 / GA) void mfma_general_kernel(const MfmaArgs A)
 {
-    __shared__ __attribute__((aligned(16))) uint8_t lds[general_lds<GA, NBG>()];
+    __shared__ __attribute__((aligned(16))) uint8_t lds[general_lds<GA>()];
     const int4 T = A.tiles[xcd_tile(blockIdx.x, A.ntiles)];
     HitSink hs = A.hs;
     hs.tag = A.S.tag;
-    mfma_general_tile<GA, NBG, DUMP>(A.S, hs, A.ds, T.x, T.y, T.z, T.w, lds);
+    mfma_general_tile<GA, DUMP>(A.S, hs, A.ds, T.x, T.y, T.z, T.w, lds);
 }
 
 }  // namespace epi
