@@ -122,8 +122,9 @@ int epi_set_hit_capacity(epi_ctx *ctx, int64_t cap);
 
 /* Tile shape and work order of a scan. The reference's only knob here is
  * the chunk size of its launches (-i, src/EpiOpenCL.c:1559-1607); these
- * settings replace it. Tiles are 64 b-SNPs wide; a FAST tile (no missing
- * call in either SNP range) is fast_rows a-SNPs high, a GENERAL tile 16. */
+ * settings replace it. A FAST tile (no missing call in either SNP range) is
+ * fast_rows a-SNPs x 64 b-SNPs; a GENERAL tile is fast_rows x 32 in the MFMA
+ * form and 16 x 64 in the plain form. */
 typedef struct epi_schedule {
     int32_t fast_rows;  /* MFMA form: 64 or 32; plain form: 32 or 16 */
     int32_t super_a;    /* tiles are issued in super-tiles of super_a x super_b */
