@@ -108,9 +108,34 @@ def accuracy(s, wl, form, phen=0):
     ref = O.pairs(codes, wl.y[phen], pr)
     c = counts.reshape(-1, 9)
     rel = lambda x, r: np.abs(x - r) / np.maximum(np.abs(r), 1.0)
-    return {"pairs_checked": int(len(pr)), "counts_exact": bool((c == ref["counts"]).all()),
-            "f_relerr_max": float(rel(F.ravel(), ref["F"]).max()),
-            "f2_relerr_max": float(rel(F2.ravel(), ref["F2"]).max())}
+    rf, rf2 = rel(F.ravel(), ref["F"]), rel(F2.ravel(), ref["F2"])
+    out = {"pairs_checked": int(len(pr)), "counts_exact": bool((c == ref["counts"]).all()),
+           "f_relerr_max": float(rf.max()), "f2_relerr_max": float(rf2.max()),
+           "f_relerr_p50": float(np.percentile(rf, 50)), "f_relerr_p99": float(np.percentile(rf, 99)),
+           "f2_relerr_p50": float(np.percentile(rf2, 50)), "f2_relerr_p99": float(np.percentile(rf2, 99)),
+           "relerr_policy": "|x - x64| / max(|x64|, 1)"}
+    # hit decisions of the block (src/EpiOpenCL.c:251, :487-489) at the workload's thresholds and at the
+    # block's own 90th percentiles of F and F2 (so that there are hits to compare); a differing pair
+    # must be a tie, |F - t| / t <= 1e-5
+    nfac = (ref["counts"] > 0).sum(1)
+    nn = ref["counts"].sum(1)
+    ok = (nfac >= 2) & (nn - nfac >= 1)
+    def hits(f, f2, t1, t2):
+        if wl.test == "f":
+            return ok & (f > t1)
+        return ok & ((f > t1) | (f2 > t2)) & (nfac > 6)
+    fin = np.isfinite(ref["F"]) & np.isfinite(ref["F2"])
+    levels = [("workload", wl.t1, wl.t2),
+              ("block p90", float(np.percentile(ref["F"][fin], 90)), float(np.percentile(ref["F2"][fin], 90)))]
+    out["hit_sets"] = []
+    for name, t1, t2 in levels:
+        g, r = hits(F.ravel(), F2.ravel(), t1, t2), hits(ref["F"], ref["F2"], t1, t2)
+        d = g != r
+        tie = (np.abs(ref["F"] - t1) <= 1e-5 * t1) | (np.abs(ref["F2"] - t2) <= 1e-5 * t2)
+        out["hit_sets"].append({"thresholds": name, "t1": t1, "t2": t2, "hits_gpu": int(g.sum()),
+                                "hits_oracle": int(r.sum()), "differ": int(d.sum()),
+                                "differ_not_tie": int((d & ~tie).sum())})
+    return out
 
 
 def run_c2(E, s, steps=3):
