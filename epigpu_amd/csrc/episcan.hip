@@ -11,7 +11,8 @@
 //     yc    [Kp]    float  centred phenotype scaled by 2^-ybits to |yc| < 1 (0 where missing)
 //     vbits [Kw]    uint32 phenotype-present bits
 //     marg  [nsnp]  Marg   per-SNP class counts and sums
-// Kp = nid rounded up to 256, Kw = Kp / 32; padding individuals are absent.
+// Kp = nid rounded up to 256 (the plain kernel's stage), Kw = Kp / 32; padding
+// individuals are absent. The MFMA K loops stop at nid rounded up to 128.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -395,6 +396,7 @@ MfmaScan mfma_scan(const Dataset &d, int r, int tag)
     S.marg = p.marg;
     S.nsnp = d.nsnp;
     S.Kp = d.Kp;
+    S.Kl = (d.nid + KC - 1) / KC * KC;
     S.nV = p.nV;
     S.tag = tag;
     S.qV = p.qV;

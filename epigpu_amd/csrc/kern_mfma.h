@@ -56,7 +56,8 @@ struct MfmaScan {                // one dataset x one phenotype
     const int8_t *wts;           // [NWP][Kp] for this phenotype
     const Marg *marg;            // [nsnp] for this phenotype
     int nsnp;
-    int Kp;                      // = 32 Kw
+    int Kp;                      // = 32 Kw: row stride of wts, plane stride of bits (32 Kp / 32 dwords)
+    int Kl;                      // individuals the K loop covers: nid rounded up to KC (<= Kp)
     int nV;                      // present phenotypes
     int tag;                     // written into the hits
     long long qV;                // sum q over present
@@ -214,20 +215,20 @@ struct SetTag {
     static constexpr int v = S;
 };
 
-// The K loop of a tile over Kp individuals (a multiple of 2 KC) in stages
+// The K loop of a tile over Kl individuals (a multiple of KC) in stages
 // of KC through two LDS buffers. Within a stage the operands of the next
 // 32-individual step are read from LDS (fetch) while the MFMAs of the
 // current one run (issue); the next stage is written to the other buffer
 // mid-stage and one barrier per stage publishes it. NMF is the number of
 // MFMAs per step (per issue).
 template <class St, class Ops, int NMF, class Fetch, class Issue>
-__device__ __forceinline__ void k_loop(St &sg, uint8_t *lds, int stage_bytes, int Kp, Fetch fetch, Issue issue)
+__device__ __forceinline__ void k_loop(St &sg, uint8_t *lds, int stage_bytes, int Kl, Fetch fetch, Issue issue)
 {
     Ops o0, o1 = {};
     sg.template load<0>(0);
-    sg.template load<1>(KC);
+    sg.template load<1>(KC < Kl ? KC : 0);
     sg.template store<0>(lds);
-    sg.template load<0>(2 * KC < Kp ? 2 * KC : 0);
+    sg.template load<0>(2 * KC < Kl ? 2 * KC : 0);
     __syncthreads();
     // One stage per region between barriers: step 3 of the previous stage
     // (o1; zero operands before the first stage) and steps 0-2 of this one
@@ -241,7 +242,7 @@ __device__ __forceinline__ void k_loop(St &sg, uint8_t *lds, int stage_bytes, in
         issue(o0);
         fetch(cur, 2, o0);
         sg.template store<S>(nxt);
-        sg.template load<S>(kc + 3 * KC < Kp ? kc + 3 * KC : Kp - KC);
+        sg.template load<S>(kc + 3 * KC < Kl ? kc + 3 * KC : Kl - KC);
         issue(o1);
         fetch(cur, 3, o1);
         issue(o0);
@@ -262,10 +263,14 @@ __device__ __forceinline__ void k_loop(St &sg, uint8_t *lds, int stage_bytes, in
         // the next stage was written by every wave during this one
         __syncthreads();
     };
-    for (int k0 = 0; k0 < Kp; k0 += 2 * KC) {
+    int k0 = 0;
+    for (; k0 + 2 * KC <= Kl; k0 += 2 * KC) {
         body(SetTag<1>{}, lds, lds + stage_bytes, k0);
         body(SetTag<0>{}, lds + stage_bytes, lds, k0 + KC);
     }
+    // an odd stage count ends with one stage in buffer 0; what it stages
+    // for a next stage lands in buffer 1 and is never read
+    if (k0 < Kl) body(SetTag<1>{}, lds, lds + stage_bytes, k0);
     issue(o1);
 }
 
@@ -275,13 +280,13 @@ __device__ __forceinline__ void k_loop(St &sg, uint8_t *lds, int stage_bytes, in
 // AND has read it (the issue order decides how early). For tiles whose
 // accumulators leave no room for a second operand set.
 template <class St, class Ops, int NMF, class Fetch, class Issue>
-__device__ __forceinline__ void k_loop1(St &sg, uint8_t *lds, int stage_bytes, int Kp, Fetch fetch, Issue issue)
+__device__ __forceinline__ void k_loop1(St &sg, uint8_t *lds, int stage_bytes, int Kl, Fetch fetch, Issue issue)
 {
     Ops o = {};
     sg.template load<0>(0);
-    sg.template load<1>(KC);
+    sg.template load<1>(KC < Kl ? KC : 0);
     sg.template store<0>(lds);
-    sg.template load<0>(2 * KC < Kp ? 2 * KC : 0);
+    sg.template load<0>(2 * KC < Kl ? 2 * KC : 0);
     __syncthreads();
     auto body = [&](auto tag, const uint8_t *cur, uint8_t *nxt, int kc) {
         constexpr int S = decltype(tag)::v;
@@ -292,7 +297,7 @@ __device__ __forceinline__ void k_loop1(St &sg, uint8_t *lds, int stage_bytes, i
         issue(o);
         fetch(cur, 2, o);
         sg.template store<S>(nxt);
-        sg.template load<S>(kc + 3 * KC < Kp ? kc + 3 * KC : Kp - KC);
+        sg.template load<S>(kc + 3 * KC < Kl ? kc + 3 * KC : Kl - KC);
         issue(o);
         fetch(cur, 3, o);
 #pragma unroll
@@ -306,10 +311,14 @@ __device__ __forceinline__ void k_loop1(St &sg, uint8_t *lds, int stage_bytes, i
         // the next stage was written by every wave during this one
         __syncthreads();
     };
-    for (int k0 = 0; k0 < Kp; k0 += 2 * KC) {
+    int k0 = 0;
+    for (; k0 + 2 * KC <= Kl; k0 += 2 * KC) {
         body(SetTag<1>{}, lds, lds + stage_bytes, k0);
         body(SetTag<0>{}, lds + stage_bytes, lds, k0 + KC);
     }
+    // an odd stage count ends with one stage in buffer 0; what it stages
+    // for a next stage lands in buffer 1 and is never read
+    if (k0 < Kl) body(SetTag<1>{}, lds, lds + stage_bytes, k0);
     issue(o);
 }
 
@@ -383,7 +392,7 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
         }
     };
 
-    k_loop<decltype(sg), Ops, 2 * 4>(sg, lds, STAGE, Kp, fetch, issue);
+    k_loop<decltype(sg), Ops, 2 * 4>(sg, lds, STAGE, A.Kl, fetch, issue);
 
     // Per-SNP class marginals of the tile, staged in LDS (the operand buffers
     // are free once every wave has left the K loop).
@@ -553,7 +562,7 @@ __device__ __forceinline__ void mfma_general_tile(const MfmaScan &A, const HitSi
         accz = __builtin_amdgcn_mfma_i32_32x32x32_i8(Az, o.b[2], accz, 0, 0, 0);
     };
 
-    k_loop1<decltype(sg), Ops, 10>(sg, lds, STAGE, Kp, fetch, issue);
+    k_loop1<decltype(sg), Ops, 10>(sg, lds, STAGE, A.Kl, fetch, issue);
 
     // Epilogue: lane holds pairs (a = a0 + 8w + 4hf + j, b = b0 + m), j = 0..3;
     // acc[g][h][4j + w] = X[g][h][w], accz[4j + w] = X[NM][NM][Z_w].
