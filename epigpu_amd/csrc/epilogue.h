@@ -152,6 +152,12 @@ __device__ __forceinline__ bool screen32(const int n[9], const float s[9], float
     // F > t  <=>  ssb * d2 > t * d1 * ssw
     if (ssb * (float)d2 > t1s * (float)d1 * ssw) return true;
     if (test == 'f') return false;
+    // SSI <= SSB: with e the cell deviations m_c - mbar and <x,y> = sum n_c x_c y_c, the
+    // row and column terms are the projections Pr e, Pc e onto row- and column-constant
+    // vectors, and SSI = |e - Pr e - Pc e|^2 = |e|^2 - |Pr e|^2 - |Pc e|^2 + 2 <Pr e, Pc e>
+    // <= |e|^2 = SSB. Hence F2 <= d1 F / 4, and no F2 hit is possible unless
+    // SSB d2 / 4 > t2 SSW, which almost no pair meets: skip the F2 terms.
+    if (!(0.25f * ssb * (float)d2 > t2s * ssw)) return false;
     float R[3], C[3];
 #pragma unroll
     for (int r = 0; r < 3; r++) {

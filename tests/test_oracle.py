@@ -36,6 +36,32 @@ def test_oracle_F2_matches_two_way_decomposition():
         assert abs(r["F"][0] - F) <= 1e-9 * max(1.0, abs(F))
 
 
+def test_interaction_ss_bounded_by_between_cells_ss():
+    """SSI <= SSB for every table, balanced or not: SSI is |e - Pr e - Pc e|^2 for the cell deviations e
+    and its n-weighted projections on row- and column-constant vectors, which is |e|^2 - |Pr e|^2 -
+    |Pc e|^2 + 2<Pr e, Pc e> <= |e|^2 = SSB. Hence F2 <= df1 * F / 4, the bound the GPU screen uses to
+    skip the F2 terms. Checked on unbalanced, heavy-tailed and missing-data tables, with equality
+    approached when the deviations are pure interaction."""
+    rng = np.random.default_rng(11)
+    worst = 0.0
+    for trial in range(30):
+        nid = int(rng.integers(30, 300))
+        maf = rng.uniform(0.02, 0.5, 24)
+        codes = ((rng.random((24, nid)) < maf[:, None]).astype(np.int8)
+                 + (rng.random((24, nid)) < maf[:, None]).astype(np.int8))
+        codes[rng.random((24, nid)) < 0.05] = 3
+        y = (rng.standard_t(2, nid) if trial % 2 else rng.standard_normal(nid)).astype(np.float32)
+        if trial % 3 == 0:  # pure interaction: y follows (g_a - 1)(g_b - 1)
+            y = y * 0.01 + ((codes[0].astype(np.float32) - 1) * (codes[12].astype(np.float32) - 1))
+        aa, bb = np.meshgrid(np.arange(12), np.arange(12, 24), indexing="ij")
+        r = O.pairs(codes, y, np.stack([aa.ravel(), bb.ravel()], 1).astype(np.int32))
+        ok = (r["nfac"] >= 2) & np.isfinite(r["F"]) & np.isfinite(r["F2"]) & (r["F"] > 0)
+        ratio = r["F2"][ok] / (r["df1"][ok] * r["F"][ok])
+        assert ratio.max() <= 0.25 * (1 + 1e-9)
+        worst = max(worst, float(ratio.max()))
+    assert worst > 0.2  # the bound is nearly attained, so it is not loose by construction
+
+
 def test_oracle_counts_brute_force():
     rng = np.random.default_rng(3)
     for _ in range(20):
