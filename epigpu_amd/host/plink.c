@@ -343,6 +343,10 @@ int epi_read_ped_map(const char *pedfile, const char *mapfile, epi_plink *d)
     while ((n = next_line(&L)) >= 0) {
         const char *p = L.buf, *end = L.buf + n, *t;
         size_t l;
+        if (i >= d->nid) {  /* the file grew between the two passes */
+            rc = fail(EPIH_ERR_FORMAT, "%s changed while it was read", pedfile);
+            goto out2;
+        }
         if ((rc = parse_person(&p, end, &d->dat[i], L.no))) goto out2;
         uint8_t *row = blk + (size_t)nr * d->nsnp;
         for (int s = 0; s < d->nsnp; s++) {
@@ -378,6 +382,10 @@ int epi_read_ped_map(const char *pedfile, const char *mapfile, epi_plink *d)
         }
     }
     if (nr) scatter_block(d, blk, i - nr, nr);
+    if (i != d->nid) {
+        rc = fail(EPIH_ERR_FORMAT, "%s changed while it was read", pedfile);
+        goto out2;
+    }
     if ((rc = recode_all(d, het, NULL))) goto out2;
     if ((rc = read_map(mapfile, d))) goto out2;
     rc = build_chromosomes(d);
