@@ -212,3 +212,38 @@ def test_single_scan_autotune(scanner, batch_data):
         assert np.abs(g["F64"] - ref["F64"]).max(initial=0) <= 1e-12 * max(1.0, np.abs(ref["F64"]).max(initial=0))
     finally:
         scanner.set_schedule("mfma", **default)
+
+
+@pytest.mark.parametrize("form", ["plain", "mfma"])
+def test_batch_safe_mode_matches_single_scans(scanner, batch_data, form):
+    """Safe mode in the batch (-9 a phenotype value, test 'i' without nfac > 6) equals safe single scans."""
+    import epigpu_amd as E
+    b = E.Batch(scanner)
+    data = []
+    for d, (nid, words, phen, starts, sizes) in enumerate(batch_data[:3]):
+        ph = phen.copy()
+        ph[:, ::7] = -9.0  # -9 is a value here
+        data.append((nid, words, ph, starts, sizes))
+        di = b.add_dataset(words, nid, starts, sizes)
+        b.add_phenotypes(di, ph, nine_is_value=True)
+    st = b.scan(test="i", form=form, t1=T1, t2=T2, safe=True)
+    assert not st["overflow"]
+    s = 0
+    for nid, words, ph, starts, sizes in data:
+        scanner.set_genotypes(words, nid)
+        scanner.set_phenotypes(ph, nine_is_value=True)
+        scanner.set_chromosomes(starts, sizes)
+        for r in range(len(ph)):
+            g = b.hits(s)
+            one = scanner.scan(test="i", form=form, t1=T1, t2=T2, phen=r, safe=True)
+            assert len(g["hits"]) == len(one["hits"]), f"scan {s}: {len(g['hits'])} != {len(one['hits'])}"
+            assert (g["hits"]["snp1"] == one["hits"]["snp1"]).all() and (g["hits"]["snp2"] == one["hits"]["snp2"]).all()
+            assert np.abs(g["F64"] - one["F64"]).max(initial=0) <= 1e-12 * max(1.0, np.abs(one["F64"]).max(initial=0))
+            s += 1
+    # and the safe batch differs from the normal one on this data (-9 counts as a value)
+    normal = E.Batch(scanner)
+    for nid, words, ph, starts, sizes in data:
+        normal.add_phenotypes(normal.add_dataset(words, nid, starts, sizes), ph)
+    assert normal.scan(test="i", form=form, t1=T1, t2=T2)["nhits"] != st["nhits"]
+    normal.close()
+    b.close()
