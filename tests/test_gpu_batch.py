@@ -53,10 +53,12 @@ def _make_batch(scanner, data):
     return b, scans
 
 
-@pytest.mark.parametrize("form", ["plain", "mfma"])
-def test_batch_matches_single_scans(scanner, batch_data, form):
+@pytest.mark.parametrize("form,rows", [("plain", 32), ("plain", 16), ("mfma", 32), ("mfma", 64)])
+def test_batch_matches_single_scans(scanner, batch_data, form, rows):
+    """Both tile heights of each form (MFMA 64: 512-thread workgroups, 64 x 32 GENERAL tiles)."""
     b, scans = _make_batch(scanner, batch_data)
-    print(f"batch form={form} scans={len(scans)}", flush=True)
+    b.set_schedule(form, fast_rows=rows)
+    print(f"batch form={form} rows={rows} scans={len(scans)}", flush=True)
     st = b.scan(test="i", form=form, t1=T1, t2=T2)
     assert not st["overflow"]
     total = 0
