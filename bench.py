@@ -138,6 +138,30 @@ def accuracy(s, wl, form, phen=0):
     return out
 
 
+def stress_hit_sets(E, s, wl, p=1e-4):
+    """The whole workload scanned by both kernel forms at thresholds for p < 1e-4 (F^-1(1-p; 8 and 4, N-9)),
+    where the null alone gives about p * pairs hits per test: the two forms are independent implementations
+    (INT8 MFMA on exact fixed-point sums; AND+popcount with FP32 sums) whose FP64 decisions must agree apart
+    from pairs within 1e-5 of a threshold."""
+    t1 = float(np.float32(E.f_isf(p, 8, wl.nid - 9)))
+    t2 = float(np.float32(E.f_isf(p, 4, wl.nid - 9)))
+    got = {}
+    for form in ("mfma", "plain"):
+        r = s.scan(test=wl.test, form=form, t1=t1, t2=t2)
+        if r["overflow"]:  # grow the hit buffer to the counted total and scan again
+            s.set_hit_capacity(r["nhits"] + 1024)
+            r = s.scan(test=wl.test, form=form, t1=t1, t2=t2)
+        assert not r["overflow"]
+        h = r["hits"]
+        got[form] = {(int(a), int(b)): (f, f2) for a, b, f, f2 in zip(h["snp1"], h["snp2"], r["F64"], r["F2_64"])}
+    diff = set(got["mfma"]) ^ set(got["plain"])
+    def tie(pr):
+        f, f2 = got["mfma"].get(pr) or got["plain"][pr]
+        return abs(f - t1) <= 1e-5 * t1 or abs(f2 - t2) <= 1e-5 * t2
+    return {"p": p, "t1": t1, "t2": t2, "hits_mfma": len(got["mfma"]), "hits_plain": len(got["plain"]),
+            "differ": len(diff), "differ_not_tie": sum(not tie(pr) for pr in diff)}
+
+
 def run_c2(E, s, steps=3):
     """c2: N=4000, M=20000 over 2 chromosomes, 1% missing calls, 2% NaN phenotypes, test i: the plain popcount
     kernel's configuration (every tile is GENERAL), both forms, autotuned schedules, oracle check of each."""
@@ -416,6 +440,9 @@ def main():
     if not args.no_check:
         out["check"] = accuracy(s, wl, args.form)
         log(f"[bench] oracle check: {out['check']}")
+        if not args.no_plain and args.form == "mfma":
+            out["stress_hit_sets"] = stress_hit_sets(E, s, wl)
+            log(f"[bench] stress hit sets: {out['stress_hit_sets']}")
     out["config"]["schedule"] = s.schedule(args.form)
     if tuned:
         out["autotune"] = tuned
