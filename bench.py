@@ -184,6 +184,8 @@ def run_c2(E, s, steps=3):
                      "schedule": s.schedule(form), "roofline": roofline(form, wl, km), "check": accuracy(s, wl, form)}
         log(f"[bench c2] {form}: {km:.1f} ms kernel, {out[form]['pairs_per_s']:.3e} pairs/s")
     out["mfma_over_plain"] = out["plain"]["kernel_ms"] / out["mfma"]["kernel_ms"]
+    out["stress_hit_sets"] = stress_hit_sets(E, s, wl)
+    log(f"[bench c2] stress hit sets: {out['stress_hit_sets']}")
     return out
 
 
