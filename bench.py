@@ -338,7 +338,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: the scan runs on one MI355X per process and this build has no "
-              "multi-GPU path; pairs would shard as independent tile ranges over several processes). Use --gpus 1.",
+              "multi-GPU path (pairs would shard as independent tile ranges over several processes). Use --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 
