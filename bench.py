@@ -56,15 +56,23 @@ def cpu_baseline(wl, seconds, threads):
     t0 = time.perf_counter()
     npairs, nh = O.ref32_scan_tiles(wl.words, wl.nid, wl.y[0], tl, T, wl.test, wl.t1, wl.t2)
     dt = time.perf_counter() - t0
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": npairs / dt, "unit": "pair-tests/s", "cores": threads, "kind": "port",
-            "sample": f"{len(tl)} seeded 128x128 off-diagonal tiles of the workload ({npairs} pairs, {dt:.1f} s)"}
+            "sample": f"{len(tl)} seeded 128x128 off-diagonal tiles of the workload ({npairs} pairs, {dt:.1f} s)",
+            "cpu_model": model, "host_cpus": os.cpu_count()}
 
 
 # HBM bytes per launch of the dominant kernel, from the committed rocprofv3 summary of the same workload and
-# schedule (FETCH_SIZE + WRITE_SIZE, each scaled by its factor on a copy of known size).
-# Profiled at FAST tile height 32 with 16 x 2 super-tiles; other super-tile shapes of the same tile height
-# differ only in L2 reuse, and HBM runs at about 1% of its bandwidth either way.
-# rocprofv3 summaries holding the HBM bytes per launch of the headline kernel, per FAST tile height
+# tile height (FETCH_SIZE + WRITE_SIZE, each scaled by its factor on a copy of known size). Both heights were
+# profiled with 16 x 2 super-tiles; other super-tile shapes of the same height differ only in L2 reuse, and HBM
+# runs at about 1% of its bandwidth either way.
 PROFILED = {("c3", "mfma", 64): ("profiles/mfma_fast_kernel_c3.md", "TRAFFIC_BYTES_PER_LAUNCH"),
             ("c3", "mfma", 32): ("profiles/mfma_fast_kernel_c3.md", "TRAFFIC_BYTES_PER_LAUNCH_32")}
 
