@@ -562,6 +562,21 @@ int check_params(epi_ctx *c, const epi_scan_params *p)
     return EPI_OK;
 }
 
+// A launch covers at most 2^31 work-items (the runtime counts the global size in 32 bits; beyond 2^32 it
+// would launch only the wrapped remainder), so a long tile list goes out as several launches, each over its
+// own slice of the list and a whole number of 256-workgroup groups (the XCD mapping of xcd_tile).
+#define LAUNCH_TILES(KERNEL, THREADS, ARGS, FIRST, N)                                                   \
+    do {                                                                                                 \
+        const int chunk_ = (int)((1u << 31) / (unsigned)(THREADS)) & ~255;                              \
+        for (int t0_ = 0; t0_ < (N); t0_ += chunk_) {                                                    \
+            (ARGS).tiles = (FIRST) + t0_;                                                                \
+            (ARGS).ntiles = std::min(chunk_, (N) - t0_);                                                 \
+            hipLaunchKernelGGL(KERNEL, dim3((ARGS).ntiles), dim3(THREADS), 0, c->stream, (ARGS));        \
+            HIPCHK(c, hipGetLastError());                                                                \
+            c->last_launches++;                                                                          \
+        }                                                                                                \
+    } while (0)
+
 // Launch the pair kernels of one scan over the uploaded tile lists (fast tiles first).
 template <bool DUMP>
 int launch_pairs(epi_ctx *c, const epi_scan_params *p, const epi_schedule &sc, const TileLists &tl,
@@ -576,50 +591,23 @@ int launch_pairs(epi_ctx *c, const epi_scan_params *p, const epi_schedule &sc, c
         A.S = mfma_scan(d, p->phen, p->phen);
         A.hs = hs;
         A.ds = ds;
-        if (nf) {
-            A.tiles = c->tiles;
-            A.ntiles = nf;
-            if (sc.fast_rows == 64)
-                hipLaunchKernelGGL((mfma_fast_kernel<64, DUMP>), dim3(nf), dim3(fast_threads<64>()), 0, c->stream, A);
-            else
-                hipLaunchKernelGGL((mfma_fast_kernel<32, DUMP>), dim3(nf), dim3(fast_threads<32>()), 0, c->stream, A);
-            HIPCHK(c, hipGetLastError());
-            c->last_launches++;
-        }
-        if (ng) {
-            A.tiles = c->tiles + nf;
-            A.ntiles = ng;
-            if (sc.fast_rows == 64)
-                hipLaunchKernelGGL((mfma_general_kernel<64, DUMP>), dim3(ng), dim3(general_threads<64>()), 0,
-                                   c->stream, A);
-            else
-                hipLaunchKernelGGL((mfma_general_kernel<32, DUMP>), dim3(ng), dim3(general_threads<32>()), 0,
-                                   c->stream, A);
-            HIPCHK(c, hipGetLastError());
-            c->last_launches++;
+        if (sc.fast_rows == 64) {
+            LAUNCH_TILES((mfma_fast_kernel<64, DUMP>), fast_threads<64>(), A, c->tiles, nf);
+            LAUNCH_TILES((mfma_general_kernel<64, DUMP>), general_threads<64>(), A, c->tiles + nf, ng);
+        } else {
+            LAUNCH_TILES((mfma_fast_kernel<32, DUMP>), fast_threads<32>(), A, c->tiles, nf);
+            LAUNCH_TILES((mfma_general_kernel<32, DUMP>), general_threads<32>(), A, c->tiles + nf, ng);
         }
     } else {
         PlainArgs P;
         P.S = plain_scan(d, p->phen, p->phen);
         P.hs = hs;
         P.ds = ds;
-        if (nf) {
-            P.tiles = c->tiles;
-            P.ntiles = nf;
-            if (sc.fast_rows == 32)
-                hipLaunchKernelGGL((plain_kernel<2, 4, true, DUMP>), dim3(nf), dim3(256), 0, c->stream, P);
-            else
-                hipLaunchKernelGGL((plain_kernel<1, 4, true, DUMP>), dim3(nf), dim3(256), 0, c->stream, P);
-            HIPCHK(c, hipGetLastError());
-            c->last_launches++;
-        }
-        if (ng) {
-            P.tiles = c->tiles + nf;
-            P.ntiles = ng;
-            hipLaunchKernelGGL((plain_kernel<1, 4, false, DUMP>), dim3(ng), dim3(256), 0, c->stream, P);
-            HIPCHK(c, hipGetLastError());
-            c->last_launches++;
-        }
+        if (sc.fast_rows == 32)
+            LAUNCH_TILES((plain_kernel<2, 4, true, DUMP>), 256, P, c->tiles, nf);
+        else
+            LAUNCH_TILES((plain_kernel<1, 4, true, DUMP>), 256, P, c->tiles, nf);
+        LAUNCH_TILES((plain_kernel<1, 4, false, DUMP>), 256, P, c->tiles + nf, ng);
     }
     return EPI_OK;
 }

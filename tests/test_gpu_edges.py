@@ -131,3 +131,32 @@ def test_mfma_64_row_tiles_dump(scanner, nid):
         assert rel_err(F2[ok], ref["F2"][ok]).max() <= TOL_F
     finally:
         scanner.set_schedule("mfma", **default)
+
+
+def test_single_scan_beyond_one_launch(scanner):
+    """More tiles than one launch may cover (HIP counts a launch's work-items in 32 bits): 200,000 SNPs with
+    missing calls give about 9.8 million 64 x 32 GENERAL tiles of 512 threads, 5e9 work-items. The single
+    scan must split its launches and find exactly the hits of the persistent batch launch of the same data."""
+    import epigpu_amd as E
+    nid, nsnp = 100, 200000
+    words, y = make_dataset(0xB16, nid, nsnp, maf=(0.1, 0.5), gmiss=0.001)
+    default = scanner.schedule("mfma")
+    try:
+        scanner.set_genotypes(words, nid)
+        scanner.set_phenotypes(y)
+        scanner.set_chromosomes([0], [nsnp])
+        scanner.set_schedule("mfma", fast_rows=64, super_a=16, super_b=2)
+        print("single scan beyond one launch", flush=True)
+        one = scanner.scan(test="f", form="mfma", t1=7.5)
+        assert one["launches"] >= 2 and not one["overflow"]
+        assert one["pairs"] == nsnp * (nsnp - 1) // 2
+        b = E.Batch(scanner)
+        b.add_phenotypes(b.add_dataset(words, nid, [0], [nsnp]), y)
+        b.set_schedule("mfma", fast_rows=64, super_a=16, super_b=2)
+        st = b.scan(test="f", form="mfma", t1=7.5)
+        g = b.hits(0)
+        b.close()
+        assert not st["overflow"] and g["nhits"] == one["nhits"] > 100
+        assert (g["hits"]["snp1"] == one["hits"]["snp1"]).all() and (g["hits"]["snp2"] == one["hits"]["snp2"]).all()
+    finally:
+        scanner.set_schedule("mfma", **default)
