@@ -74,10 +74,11 @@ def test_rect_ragged_nid(scanner, form, nid):
 
 @pytest.mark.parametrize("form", ["plain", "mfma"])
 @pytest.mark.parametrize("gmiss", [0.0, 0.01])
-def test_rect_large_nid(scanner, form, gmiss):
-    """N = 20,001 (c5's scale; an odd number of 128-individual stages): FAST tiles without missing calls,
-    GENERAL tiles with them; counts exact and F / F2 within tolerance at long K loops."""
-    nid, nsnp = 20001, 72
+@pytest.mark.parametrize("nid", [20001, 100003])
+def test_rect_large_nid(scanner, form, gmiss, nid):
+    """N = 20,001 (c5's scale) and 100,003 (odd numbers of 128-individual stages): FAST tiles without missing
+    calls, GENERAL tiles with them; counts exact and F / F2 within tolerance at long K loops."""
+    nsnp = 72
     words, y = make_dataset(0x5EED + int(gmiss * 100), nid, nsnp, maf=(0.01, 0.5), gmiss=gmiss, pmiss=0.01)
     print(f"rect large form={form} nid={nid} gmiss={gmiss}", flush=True)
     check_rect(scanner, words, y, nid, 0, nsnp, 0, nsnp, form)
