@@ -233,3 +233,27 @@ def test_plink_ingest_then_scan(tmp_path):
         x = ref[(ref["snp1"] == pr[0]) & (ref["snp2"] == pr[1])]
         assert len(x) == 0 or min(abs(x["F"][0] - 1.5), abs(x["F2"][0] - 1.5)) < 1e-5 * 1.5
     assert len(got ^ want) <= 2
+
+
+def test_empty_chromosome_blocks(tmp_path):
+    """A chromosome left with no SNPs (as cleaning can leave one) keeps its blocks in the output, each with
+    0 interactions, and the other blocks' hits are the oracle's."""
+    import epigpu_amd as E
+    from epigpu_amd import scan as S
+    nid, sizes = 400, [60, 0, 45]
+    nsnp = sum(sizes)
+    words, y = make_dataset(77, nid, nsnp, maf=(0.05, 0.5), gmiss=0.01, pmiss=0.02)
+    y = np.where(np.isnan(y), np.float32(-9.0), y).astype(np.float32)
+    ped, mp, chrs = S.make_records(nid, nsnp, sizes, y)
+    out = str(tmp_path / "out.txt")
+    rep = E.perform_scan(words, nid, ped, mp, chrs, out, test="i", t1=T1, t2=T2)
+    header, blocks, rows = parse_output(out)
+    assert [b for b, _ in blocks] == [(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2)]
+    assert dict(blocks)[(1, 0)] == dict(blocks)[(1, 1)] == dict(blocks)[(2, 1)] == 0
+    n_ref, ref = O.scan(O.unpack(words, nid), y, [0, 60, 60], sizes, "i", T1, T2)
+    got = {(a, b) for rs in rows.values() for a, b, *_ in rs}
+    want = set(zip(ref["snp1"].tolist(), ref["snp2"].tolist()))
+    assert n_ref > 10 and rep["total_hits"] == len(got)
+    for pr in got ^ want:  # only threshold ties may differ
+        x = ref[(ref["snp1"] == pr[0]) & (ref["snp2"] == pr[1])]
+        assert len(x) == 0 or min(abs(x["F"][0] - T1) / T1, abs(x["F2"][0] - T2) / T2) < 1e-5
