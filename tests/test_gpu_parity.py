@@ -234,3 +234,47 @@ def test_safe_mode_scan(scanner, form):
     orc = O.pairs(codes, y, np.stack(np.nonzero(sel), 1), safe=True)
     assert (counts[sel] == orc["counts"]).all()
     scanner.set_phenotypes(y)  # back to normal -9 handling for later tests
+
+
+@pytest.mark.parametrize("form,gmiss", [("plain", 0.01), ("mfma", 0.0), ("mfma", 0.01)])
+def test_class_relabel_invariance_full_n(scanner, form, gmiss):
+    """At N = 8,000 (the 50,000-SNP workload's cohort), swapping classes 0 and 2 on a third of the SNPs (what a
+    different first-seen homozygote does) moves each pair's cells to (s_a(g1), s_b(g2)) bit-exactly and leaves F
+    and F2 unchanged: the 8df F is a one-way ANOVA over cells and the interaction SS sums over rows and columns.
+    A kernel that mixes up class planes, weight rows or the derived class-2 cell breaks the cell map. MFMA runs
+    both tile kinds (no missing calls: 64-row FAST tiles; 1 % missing: GENERAL tiles)."""
+    import epigpu_amd as E
+    nid, nsnp = 8000, 640
+    words, y = make_dataset(41, nid, nsnp, maf=(0.02, 0.5), gmiss=gmiss, pmiss=0.01)
+    codes = E.unpack(words, nid)
+    flip = np.random.default_rng(5).random(nsnp) < 1 / 3
+    hom = (codes == 0) | (codes == 2)
+    sw = np.where(flip[:, None] & hom, 2 - codes, codes).astype(np.int8)
+    print(f"relabel form={form} gmiss={gmiss} nid={nid} nsnp={nsnp} flipped={int(flip.sum())}", flush=True)
+    scanner.set_genotypes(words, nid)
+    scanner.set_phenotypes(y)
+    c0, s0, F0, G0 = scanner.pair_dump(0, nsnp, 0, nsnp, test="i", form=form)
+    scanner.set_genotypes(E.pack(sw), nid)
+    c1, s1, F1, G1 = scanner.pair_dump(0, nsnp, 0, nsnp, test="i", form=form)
+    aa, bb = np.meshgrid(np.arange(nsnp), np.arange(nsnp), indexing="ij")
+    sel = aa > bb
+    g1, g2 = np.arange(9) // 3, np.arange(9) % 3
+    fa, fb = flip[aa[sel]][:, None], flip[bb[sel]][:, None]
+    dst = 3 * np.where(fa, 2 - g1, g1) + np.where(fb, 2 - g2, g2)
+    rows = np.arange(dst.shape[0])[:, None]
+    mapped = np.empty_like(c0[sel])
+    mapped[rows, dst] = c0[sel]
+    assert (c1[sel] == mapped).all(), f"{form}: cell map broken in {(c1[sel] != mapped).any(1).sum()} pairs"
+    msum = np.empty_like(s0[sel])
+    msum[rows, dst] = s0[sel]
+    scale = np.abs(y[np.isfinite(y) & (y != -9)]).max()
+    if form == "mfma":
+        # both runs sum the same fixed-point phenotype values over the same individuals
+        tol_s, tol_f = (mapped + 1) * scale * 2.0 ** -22, TOL_F
+    else:
+        # each run is within the FP32 bound of the exact sums, and each F within TOL_F of the oracle
+        tol_s, tol_f = 2 * (1e-7 * nid * scale + 1e-9), 2 * TOL_F
+    assert (np.abs(s1[sel] - msum) <= tol_s).all()
+    ok = (c0[sel] > 0).sum(1) >= 2
+    assert rel_err(F1[sel][ok], F0[sel][ok]).max() <= tol_f
+    assert rel_err(G1[sel][ok], G0[sel][ok]).max() <= tol_f

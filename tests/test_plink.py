@@ -187,6 +187,30 @@ def test_tab_separated_and_round_trip(tmp_path):
     assert _bytes(tmp_path / "x.epigpu") == _bytes(tmp_path / "s.epigpu")
 
 
+@pytest.mark.parametrize("nid", [65, 131])
+def test_cleaned_round_trip_relabels_only_leading_class2_snps(tmp_path, nid):
+    """After cleaning drops individuals, a SNP's first kept homozygote can be class 2.
+    Extracting writes class 0 as the A1 homozygote, so re-reading makes that homozygote
+    class 0 again (first homozygote seen, src/readplink.c:266-334): exactly those SNPs
+    swap classes 0 and 2; every other SNP, and every missing or het call, is unchanged.
+    Sizes cross the 64-individual block boundary of the text reader, and the .bed reader
+    gives the same container."""
+    ds = make_dataset(7, nid=nid, nsnp=50, chr_sizes=(25, 25))
+    write_ped_map(ds, str(tmp_path / "a.ped"), str(tmp_path / "a.map"))
+    write_bed(ds, str(tmp_path / "b"))
+    a = E.read_plink((str(tmp_path / "a.ped"), str(tmp_path / "a.map")), clean=True,
+                     container=str(tmp_path / "a.epigpu"))
+    assert (E.read_plink(str(tmp_path / "b"), clean=True)["words"] == a["words"]).all()
+    E.write_ped_map(str(tmp_path / "a.epigpu"), str(tmp_path / "x.ped"), str(tmp_path / "x.map"))
+    r = E.read_plink((str(tmp_path / "x.ped"), str(tmp_path / "x.map")))
+    A, R = a["codes"].astype(int), r["codes"].astype(int)
+    lead = np.array([s[(s == 0) | (s == 2)][0] for s in A])
+    swapped = np.where((A == 0) | (A == 2), 2 - A, A)
+    assert (lead == 2).any()
+    assert (R[lead == 2] == swapped[lead == 2]).all()
+    assert (R[lead == 0] == A[lead == 0]).all()
+
+
 def test_random_steps_follow_srand_stream(tmp_path):
     """Imputation and the random phenotype draw the sequence rand() gives after srand(seed),
     without disturbing the process's own rand() state."""
