@@ -34,6 +34,9 @@ using namespace epi;
 namespace {
 
 constexpr int KPAD = 256;
+// The transcode grid has one row of workgroups per 32 individuals, and a grid
+// row index stays below 65536: at most 8191 K-pads of individuals.
+constexpr int MAX_NID = 65535 / (KPAD / 32) * KPAD;
 constexpr long long QMAX = 8355711;   // largest |q| with three balanced int8 limbs
 constexpr double ZSHIFT = 8388608.0;  // 2^23
 
@@ -91,6 +94,7 @@ struct epi_ctx {
     // dataset, block range, form and schedule stay the same
     int4 *tiles = nullptr;
     size_t tiles_cap = 0;
+    size_t max_tiles = (size_t)1 << 25;  // 512 MiB of int4
     TileLists tl;
     bool tl_valid = false;
     int tl_bb = 0, tl_be = 0, tl_form = -1;
@@ -203,6 +207,7 @@ void free_dataset(Dataset &d)
 int ds_set_genotypes(epi_ctx *c, Dataset &d, const int32_t *words, int nid, int nsnp)
 {
     if (!words || nid < 2 || nsnp < 2) return fail(c, EPI_ERR_ARG, "need nid >= 2, nsnp >= 2 and words");
+    if (nid > MAX_NID) return fail(c, EPI_ERR_RANGE, "nid %d above the supported %d", nid, MAX_NID);
     if (!d.ph.empty() && nid != d.nid) {  // a new cohort: phenotypes of the old one no longer apply
         for (auto &p : d.ph) free_phen(p);
         d.ph.clear();
@@ -252,6 +257,7 @@ int ds_set_phenotypes(epi_ctx *c, Dataset &d, const float *phen, int nphen, int 
     const bool nine_value = (flags & EPI_PHEN_NINE_IS_VALUE) != 0;
     auto missing = [&](float v) { return v != v || (!nine_value && v == -9.0f); };
     if (!phen || nphen < 1 || nid < 2) return fail(c, EPI_ERR_ARG, "need phenotypes");
+    if (nid > MAX_NID) return fail(c, EPI_ERR_RANGE, "nid %d above the supported %d", nid, MAX_NID);
     if (d.nid && nid != d.nid) return fail(c, EPI_ERR_ARG, "nid %d differs from genotypes' nid %d", nid, d.nid);
     const int Kp = (nid + KPAD - 1) / KPAD * KPAD, Kw = Kp / 32;
     for (auto &p : d.ph) free_phen(p);
@@ -477,10 +483,22 @@ void emit_tile(const Dataset &d, int a0, int a1, int b0, int b1, int grows, int 
     }
 }
 
-// Tiles over the chr x chr blocks [blk_begin, blk_end). Within a block the
+// Where a piecewise tile build resumes: the next chr x chr block (index
+// i(i+1)/2 + j) and, within it, the next super-tile row.
+struct TileCursor {
+    int blk = 0;
+    int sa = 0;
+};
+
+// Tiles over the chr x chr blocks [cur.blk, blk_end). Within a block the
 // tiles are emitted in super-tiles of super_a x super_b tiles so that
-// neighbouring workgroups share SNP data in L2.
-void build_tiles(const Dataset &d, int blk_begin, int blk_end, int form, const epi_schedule &sc, TileLists &out)
+// neighbouring workgroups share SNP data in L2. The build stops after the
+// super-tile row in which the list reaches max_tiles (a soft bound: one row
+// of super-tiles is never split) and leaves cur at the next row; it returns
+// true once the block range is exhausted. A block's pairs are counted with
+// its first row.
+bool build_tiles(const Dataset &d, int blk_end, int form, const epi_schedule &sc, size_t max_tiles, TileCursor &cur,
+                 TileLists &out)
 {
     const int TA = sc.fast_rows, TB = TILE_B, grows = gen_rows(form, sc), gcols = gen_cols(form, sc);
     const int nchr = (int)d.chrstart.size();
@@ -488,16 +506,22 @@ void build_tiles(const Dataset &d, int blk_begin, int blk_end, int form, const e
     for (int i = 0; i < nchr; i++) {
         for (int j = 0; j <= i; j++) {
             int blk = i * (i + 1) / 2 + j;
-            if (blk < blk_begin || blk >= blk_end) continue;
+            if (blk < cur.blk) continue;
+            if (blk >= blk_end) return true;
+            cur.blk = blk;
             int A0 = d.chrstart[i], A1 = A0 + d.chrsize[i];
             int B0 = d.chrstart[j], B1 = B0 + d.chrsize[j];
-            if (A1 <= A0 || B1 <= B0) continue;
-            for (long long a = A0; a < A1; a++) {  // pairs in this block (a > b)
-                long long lo = B0, hi = std::min<long long>(B1, a);
-                if (hi > lo) out.pairs += hi - lo;
+            if (A1 <= A0 || B1 <= B0) {
+                cur.sa = 0;
+                continue;
             }
+            if (cur.sa == 0)
+                for (long long a = A0; a < A1; a++) {  // pairs in this block (a > b)
+                    long long lo = B0, hi = std::min<long long>(B1, a);
+                    if (hi > lo) out.pairs += hi - lo;
+                }
             int nta = (A1 - A0 + TA - 1) / TA, ntb = (B1 - B0 + TB - 1) / TB;
-            for (int sa = 0; sa < nta; sa += SA)
+            for (int sa = cur.sa; sa < nta; sa += SA) {
                 for (int sb = 0; sb < ntb; sb += SB)
                     for (int ta = sa; ta < std::min(nta, sa + SA); ta++)
                         for (int tb = sb; tb < std::min(ntb, sb + SB); tb++) {
@@ -505,8 +529,19 @@ void build_tiles(const Dataset &d, int blk_begin, int blk_end, int form, const e
                             int b0 = B0 + tb * TB, b1 = std::min(B1, b0 + TB);
                             emit_tile(d, a0, a1, b0, b1, grows, gcols, out);
                         }
+                if (out.fast.size() + out.gen.size() >= max_tiles && sa + SA < nta) {
+                    cur.sa = sa + SA;
+                    return false;
+                }
+            }
+            cur.sa = 0;
+            if (out.fast.size() + out.gen.size() >= max_tiles) {
+                cur.blk = blk + 1;
+                return blk + 1 >= blk_end || blk + 1 >= nchr * (nchr + 1) / 2;
+            }
         }
     }
+    return true;
 }
 
 int upload_tiles(epi_ctx *c, const TileLists &tl)
@@ -583,6 +618,8 @@ int launch_pairs(epi_ctx *c, const epi_scan_params *p, const epi_schedule &sc, c
                  const DumpSink &ds)
 {
     const Dataset &d = c->d;
+    if (tl.fast.size() + tl.gen.size() > (size_t)INT32_MAX)
+        return fail(c, EPI_ERR_RANGE, "a super-tile row of this scan exceeds 2^31 tiles");
     const int nf = (int)tl.fast.size(), ng = (int)tl.gen.size();
     c->last_launches = 0;
     const HitSink hs = make_sink(c->hits, c->nhits, c->cap, p);
@@ -675,7 +712,8 @@ int build_batch_work(epi_batch *b, int form, const epi_schedule &sched)
             if (b->scans[s].first == di) sc.push_back(s);
         if (sc.empty()) continue;
         TileLists tl;
-        build_tiles(d, 0, 1 << 30, form, sched, tl);
+        TileCursor all;
+        build_tiles(d, 1 << 30, form, sched, SIZE_MAX, all, tl);
         b->pairs += tl.pairs * (long long)sc.size();
         for (int kind = 0; kind < 2; kind++) {
             const std::vector<int4> &v = kind ? tl.gen : tl.fast;
@@ -872,7 +910,10 @@ int batch_fetch(epi_batch *b)
 }
 
 // One scan of the block range of p with schedule sc: tile list, upload,
-// pair kernels, hit count. Records the device time and counts in c.
+// pair kernels, hit count. Records the device time and counts in c. A tile
+// list of up to c->max_tiles tiles is built once and kept on the device; a
+// longer one is built, uploaded and launched in pieces of about that many
+// tiles, and the timed region then also holds the uploads of the later pieces.
 int run_scan(epi_ctx *c, const epi_scan_params *p, const epi_schedule &sc)
 {
     int rc;
@@ -880,21 +921,40 @@ int run_scan(epi_ctx *c, const epi_scan_params *p, const epi_schedule &sc)
     int nblk = nchr * (nchr + 1) / 2;
     int bb = std::max(0, p->blk_begin), be = p->blk_end <= 0 ? nblk : std::min(nblk, p->blk_end);
     TileLists &tl = c->tl;
-    if (!(c->tl_valid && c->tl_bb == bb && c->tl_be == be && c->tl_form == p->form && same_schedule(c->tl_sched, sc))) {
+    const bool cached =
+        c->tl_valid && c->tl_bb == bb && c->tl_be == be && c->tl_form == p->form && same_schedule(c->tl_sched, sc);
+    TileCursor cur;
+    cur.blk = bb;
+    bool done = cached;
+    if (!cached) {
         tl = TileLists();
-        build_tiles(c->d, bb, be, p->form, sc, tl);
+        done = build_tiles(c->d, be, p->form, sc, c->max_tiles, cur, tl);
         if ((rc = upload_tiles(c, tl))) return rc;
-        c->tl_valid = true;
-        c->tl_bb = bb;
-        c->tl_be = be;
-        c->tl_form = p->form;
-        c->tl_sched = sc;
+        if (done) {
+            c->tl_valid = true;
+            c->tl_bb = bb;
+            c->tl_be = be;
+            c->tl_form = p->form;
+            c->tl_sched = sc;
+        }
     }
     HIPCHK(c, hipMemsetAsync(c->nhits, 0, sizeof(unsigned long long), c->stream));
     DumpSink ds;
     memset(&ds, 0, sizeof(ds));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    if ((rc = launch_pairs<false>(c, p, sc, tl, ds))) return rc;
+    long long pairs = tl.pairs;
+    int launches = 0;
+    for (;;) {
+        if ((rc = launch_pairs<false>(c, p, sc, tl, ds))) return rc;
+        launches += c->last_launches;
+        if (done) break;
+        // the launched kernels read the device tile list that the next piece overwrites
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        tl = TileLists();
+        done = build_tiles(c->d, be, p->form, sc, c->max_tiles, cur, tl);
+        if ((rc = upload_tiles(c, tl))) return rc;
+        pairs += tl.pairs;
+    }
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     unsigned long long nh = 0;
     HIPCHK(c, hipMemcpyAsync(&nh, c->nhits, sizeof(nh), hipMemcpyDeviceToHost, c->stream));
@@ -902,7 +962,8 @@ int run_scan(epi_ctx *c, const epi_scan_params *p, const epi_schedule &sc)
     float ms = 0.0f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
-    c->last_pairs = tl.pairs;
+    c->last_pairs = pairs;
+    c->last_launches = launches;
     c->last_nhits = (long long)nh;
     return EPI_OK;
 }
@@ -1098,6 +1159,14 @@ int epi_set_hit_capacity(epi_ctx *c, int64_t cap)
     if (rc) return rc;
     c->cap = cap;
     c->last_nhits = 0;
+    return EPI_OK;
+}
+
+int epi_set_max_tiles(epi_ctx *c, int64_t max_tiles)
+{
+    if (!c || max_tiles < 1 || max_tiles > (int64_t)INT32_MAX / 2) return EPI_ERR_ARG;
+    c->max_tiles = (size_t)max_tiles;
+    c->tl_valid = false;
     return EPI_OK;
 }
 

@@ -153,6 +153,7 @@ def _declare(name, L):
             "epi_batch_hit_count": (ctypes.c_int, [vp, i32, _p(i64), _p(i32)]),
             "epi_batch_get_hits": (ctypes.c_int, [vp, i32, vp, vp, vp, vp, i64, _p(i64)]),
             "epi_batch_stats": (ctypes.c_int, [vp, _p(f64), _p(i64), _p(i64), _p(i32), _p(i32)]),
+            "epi_set_max_tiles": (ctypes.c_int, [vp, i64]),
             "epi_set_schedule": (ctypes.c_int, [vp, i32, _p(Schedule)]),
             "epi_autotune": (ctypes.c_int, [vp, _p(ScanParams), i32, i32, _p(Schedule), _p(f64), _p(TuneEntry), i32,
                                             _p(i32)]),
@@ -533,6 +534,10 @@ class Scanner:
 
     def set_hit_capacity(self, cap):
         self._chk(self.L.epi_set_hit_capacity(self.ctx, cap), "epi_set_hit_capacity")
+
+    def set_max_tiles(self, n):
+        """Bound on a single scan's device tile list; longer lists are built and launched in pieces."""
+        self._chk(self.L.epi_set_max_tiles(self.ctx, n), "epi_set_max_tiles")
 
     def schedule(self, form="mfma"):
         sc = Schedule()

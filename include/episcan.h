@@ -120,6 +120,15 @@ int epi_get_hits(epi_ctx *ctx, epi_hit *hits, double *F64, double *F2_64, int32_
 /* Device capacity of the hit buffer (records); default 1<<22. */
 int epi_set_hit_capacity(epi_ctx *ctx, int64_t cap);
 
+/* Bound on the device tile list of a single scan (tiles of 16 bytes; default
+ * 1<<25, 512 MiB). A scan whose block range needs more tiles (from about
+ * 260,000 to 370,000 SNPs, by tile shape) is built, uploaded and launched in
+ * pieces of about this many tiles (whole rows of super-tiles), with the
+ * same hits; its epi_last_scan_stats time then includes the later pieces'
+ * uploads. No counterpart in the reference, which launches per chunk of
+ * -i SNPs (src/EpiOpenCL.c:1564-1607). 1 <= max_tiles <= 2^30 - 1. */
+int epi_set_max_tiles(epi_ctx *ctx, int64_t max_tiles);
+
 /* Tile shape and work order of a scan. The reference's only knob here is
  * the chunk size of its launches (-i, src/EpiOpenCL.c:1559-1607); these
  * settings replace it. A FAST tile (no missing call in either SNP range) is

@@ -160,3 +160,41 @@ def test_single_scan_beyond_one_launch(scanner):
         assert (g["hits"]["snp1"] == one["hits"]["snp1"]).all() and (g["hits"]["snp2"] == one["hits"]["snp2"]).all()
     finally:
         scanner.set_schedule("mfma", **default)
+
+
+@pytest.mark.parametrize("form", ["plain", "mfma"])
+def test_scan_in_pieces_matches_one_list(scanner, form):
+    """A tile list above the context's bound is built, uploaded and launched in pieces (whole rows of
+    super-tiles, across chromosome blocks). With the bound cut to 7, 60 and 500 tiles the scan must find exactly
+    the hits and pair count of the one-list scan, for the whole range and for a block sub-range; afterwards the
+    default bound gives the one-list scan again. Some SNPs have missing calls, so FAST and GENERAL tiles mix."""
+    import epigpu_amd as E
+    nid, nsnp = 300, 1500
+    words, y = make_dataset(0x91E, nid, nsnp, maf=(0.05, 0.5), gmiss=0.0)
+    codes = E.unpack(words, nid)
+    codes[np.random.default_rng(3).random(nsnp) < 0.1, 7] = 3  # individual 7 misses a call on a tenth of the SNPs
+    words = E.pack(codes)
+    starts, sizes = [0, 700, 700, 1200], [700, 0, 500, 300]
+    scanner.set_genotypes(words, nid)
+    scanner.set_phenotypes(y)
+    scanner.set_chromosomes(starts, sizes)
+    scanner.set_hit_capacity(1 << 22)
+    key = lambda r: (r["hits"]["snp1"], r["hits"]["snp2"], r["block"])
+    try:
+        for bb, be in [(0, 0), (2, 8)]:
+            one = scanner.scan(test="i", form=form, t1=2.0, t2=2.5, blk_begin=bb, blk_end=be)
+            assert one["nhits"] > 1000 and not one["overflow"]
+            for cap in (7, 60, 500):
+                print(f"pieces form={form} blocks=[{bb},{be}) max_tiles={cap}", flush=True)
+                scanner.set_max_tiles(cap)
+                r = scanner.scan(test="i", form=form, t1=2.0, t2=2.5, blk_begin=bb, blk_end=be)
+                assert r["pairs"] == one["pairs"] and r["nhits"] == one["nhits"] and not r["overflow"]
+                assert r["launches"] > one["launches"]
+                for x, z in zip(key(r), key(one)):
+                    assert (x == z).all()
+                assert np.array_equal(r["F64"], one["F64"]) and np.array_equal(r["F2_64"], one["F2_64"])
+            scanner.set_max_tiles(1 << 25)
+            again = scanner.scan(test="i", form=form, t1=2.0, t2=2.5, blk_begin=bb, blk_end=be)
+            assert again["launches"] == one["launches"] and again["nhits"] == one["nhits"]
+    finally:
+        scanner.set_max_tiles(1 << 25)
