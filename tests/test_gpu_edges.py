@@ -198,3 +198,29 @@ def test_scan_in_pieces_matches_one_list(scanner, form):
             assert again["launches"] == one["launches"] and again["nhits"] == one["nhits"]
     finally:
         scanner.set_max_tiles(1 << 25)
+
+
+@pytest.mark.parametrize("gmiss", [0.0, 0.001])
+def test_largest_cohort(scanner, gmiss):
+    """nid = 2,096,896, the largest the transcode grid takes (65,528 rows of 32 individuals): 16,382 K-loop
+    stages per tile, limb sums up to 2.7e8 in their 32-bit accumulators. MFMA tables exact and F / F2 within
+    1e-5 of the oracle; the plain form's counts exact (its FP32 sums over 2 million values are not held to
+    1e-5). One K-pad more is refused with EPI_ERR_RANGE before anything changes."""
+    import epigpu_amd as E
+    nid, nsnp = 2096896, 40
+    words, y = make_dataset(77, nid, nsnp, gmiss=gmiss, pmiss=0.01)
+    print(f"largest cohort nid={nid} gmiss={gmiss}", flush=True)
+    F, F2, sums, ref = dump_vs_oracle(scanner, words, y, nid, "mfma")
+    ok = ref["df1"] >= 1
+    assert rel_err(F[ok], ref["F"][ok]).max() <= TOL_F
+    assert rel_err(F2[ok], ref["F2"][ok]).max() <= TOL_F
+    dump_vs_oracle(scanner, words, y, nid, "plain")
+    big = nid + 256
+    with pytest.raises(E.EpiError) as e:
+        scanner.set_genotypes(np.zeros((2, E.layout(big)[2]), np.int32), big)
+    assert e.value.code == -6
+    with pytest.raises(E.EpiError) as e:
+        scanner.set_phenotypes(np.zeros(big, np.float32))
+    assert e.value.code == -6
+    counts, _, _, _ = scanner.pair_dump(1, 2, 0, 1, form="mfma")
+    assert counts[0, 0].sum() == ref["counts"][0].sum()
