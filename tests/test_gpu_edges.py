@@ -224,3 +224,27 @@ def test_largest_cohort(scanner, gmiss):
     assert e.value.code == -6
     counts, _, _, _ = scanner.pair_dump(1, 2, 0, 1, form="mfma")
     assert counts[0, 0].sum() == ref["counts"][0].sum()
+
+
+def test_pieces_at_default_bound(scanner):
+    """400,000 SNPs with missing calls under the default schedule: 78 million 32 x 32 GENERAL tiles, which the
+    default bound of 2^25 tiles splits into three pieces. Same hits, F64 values and pair count as one 78-million-
+    tile list."""
+    nid, nsnp = 100, 400000
+    words, y = make_dataset(0x400, nid, nsnp, maf=(0.1, 0.5), gmiss=0.001)
+    scanner.set_genotypes(words, nid)
+    scanner.set_phenotypes(y)
+    scanner.set_chromosomes([0], [nsnp])
+    try:
+        print("pieces at the default bound", flush=True)
+        r = scanner.scan(test="i", form="mfma", t1=9.0, t2=14.0)
+        scanner.set_max_tiles((1 << 30) - 1)
+        print("one list", flush=True)
+        one = scanner.scan(test="i", form="mfma", t1=9.0, t2=14.0)
+    finally:
+        scanner.set_max_tiles(1 << 25)
+    assert r["pairs"] == one["pairs"] == nsnp * (nsnp - 1) // 2
+    assert r["nhits"] == one["nhits"] > 100 and not r["overflow"] and not one["overflow"]
+    assert r["launches"] > one["launches"]
+    assert (r["hits"]["snp1"] == one["hits"]["snp1"]).all() and (r["hits"]["snp2"] == one["hits"]["snp2"]).all()
+    assert np.array_equal(r["F64"], one["F64"]) and np.array_equal(r["F2_64"], one["F2_64"])
