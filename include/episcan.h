@@ -122,7 +122,7 @@ int epi_set_hit_capacity(epi_ctx *ctx, int64_t cap);
 
 /* Bound on the device tile list of a single scan (tiles of 16 bytes; default
  * 1<<25, 512 MiB). A scan whose block range needs more tiles (from about
- * 260,000 to 370,000 SNPs, by tile shape) is built, uploaded and launched in
+ * 260,000 to 520,000 SNPs, by tile shape) is built, uploaded and launched in
  * pieces of about this many tiles (whole rows of super-tiles), with the
  * same hits; its epi_last_scan_stats time then includes the later pieces'
  * uploads. No counterpart in the reference, which launches per chunk of
