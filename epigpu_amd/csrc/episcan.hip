@@ -6,11 +6,12 @@
 //   bits    [3][Kw][nsnp]  uint32  bitplanes of the two rarest classes and call present (plain form)
 //   cls     [nsnp]         int32   plain-form plane -> class map
 //   per phenotype r:
-//     wts   [7][Kp] int8   V, 3 limbs of q, 3 limbs of round(q^2/2^23)
-//     qv    [Kp]    int32  fixed-point phenotype q (0 where missing)
+//     wts   [8][Kp] int8   V, 3 limbs of q, 4 limbs of round(q^2/2^zbits)
+//     qv    [Kp]    int32  fixed-point phenotype q (0 where missing and for outliers)
 //     yc    [Kp]    float  centred phenotype scaled by 2^-ybits to |yc| < 1 (0 where missing)
 //     vbits [Kw]    uint32 phenotype-present bits
 //     marg  [nsnp]  Marg   per-SNP class counts and sums
+//     oidx, ocodes, ov     outlier individuals (at most OUT_MAX), their codes per SNP and values
 // Kp = nid rounded up to 256 (the plain kernel's stage), Kw = Kp / 32; padding
 // individuals are absent. The MFMA K loops stop at nid rounded up to 128.
 #include <hip/hip_runtime.h>
@@ -37,8 +38,18 @@ constexpr int KPAD = 256;
 // The transcode grid has one row of workgroups per 32 individuals, and a grid
 // row index stays below 65536: at most 8191 K-pads of individuals.
 constexpr int MAX_NID = 65535 / (KPAD / 32) * KPAD;
-constexpr long long QMAX = 8355711;   // largest |q| with three balanced int8 limbs
-constexpr double ZSHIFT = 8388608.0;  // 2^23
+constexpr long long QMAX = 8355711;     // largest |q| with three balanced int8 limbs
+constexpr long long ZMAX = 2139062143;  // largest |z| with four balanced int8 limbs
+// The MFMA form's phenotype range. The central SD s_c is the SD of the
+// present values without the 2 OUT_MAX farthest from the mean (a quarter of
+// them when there are fewer than 258). Among the OUT_MAX farthest, values more
+// than OUT_SD s_c from the central mean are outliers, carried exactly. The
+// encoded values must then lie within BULK_SD s_c of the centre, which bounds
+// the fixed-point quantum at BULK_SD / QMAX = 7.7e-6 s_c; a phenotype beyond
+// that is refused by MFMA scans (EPI_ERR_RANGE), and the plain form, FP32 per
+// value, takes it.
+constexpr double OUT_SD = 32.0;
+constexpr double BULK_SD = 64.0;
 
 struct PhenDev {
     int8_t *wts = nullptr;
@@ -46,7 +57,12 @@ struct PhenDev {
     float *yc = nullptr;
     uint32_t *vbits = nullptr;
     Marg *marg = nullptr;
+    int32_t *oidx = nullptr;    // [nout] outlier individuals
+    uint32_t *ocodes = nullptr; // [ceil(nout/16)][nsnp], built with the marginals
+    OutVals *ov = nullptr;
+    int nout = 0;
     int fbits = 0;  // q = round((y - centre) * 2^fbits)
+    int zbits = 0;  // Z limbs hold round(q^2 / 2^zbits)
     int ybits = 0;  // plain form: yc = (y - centre) * 2^-ybits, |yc| < 1
     double centre = 0.0;
     int nV = 0;
@@ -54,6 +70,8 @@ struct PhenDev {
     double sqV = 0.0;  // sum q^2
     double ysV = 0.0;  // sum yc
     double y2V = 0.0;  // sum yc^2
+    bool mfma_ok = true;
+    std::string mfma_why;
 };
 
 struct Dataset {
@@ -189,6 +207,16 @@ void free_phen(PhenDev &p)
     dfree(p.yc);
     dfree(p.vbits);
     dfree(p.marg);
+    dfree(p.oidx);
+    dfree(p.ocodes);
+    dfree(p.ov);
+}
+
+void drop_phenotypes(Dataset &d)
+{
+    for (auto &p : d.ph) free_phen(p);
+    d.ph.clear();
+    d.marg_ok = false;
 }
 
 void free_dataset(Dataset &d)
@@ -204,14 +232,9 @@ void free_dataset(Dataset &d)
 
 // ------------------------------------------------------ dataset preparation --
 
-int ds_set_genotypes(epi_ctx *c, Dataset &d, const int32_t *words, int nid, int nsnp)
+int ds_upload_genotypes(epi_ctx *c, Dataset &d, const int32_t *words, int nid, int nsnp)
 {
-    if (!words || nid < 2 || nsnp < 2) return fail(c, EPI_ERR_ARG, "need nid >= 2, nsnp >= 2 and words");
-    if (nid > MAX_NID) return fail(c, EPI_ERR_RANGE, "nid %d above the supported %d", nid, MAX_NID);
-    if (!d.ph.empty() && nid != d.nid) {  // a new cohort: phenotypes of the old one no longer apply
-        for (auto &p : d.ph) free_phen(p);
-        d.ph.clear();
-    }
+    if (!d.ph.empty() && nid != d.nid) drop_phenotypes(d);  // a new cohort: the old phenotypes no longer apply
     d.nid = nid;
     d.nsnp = nsnp;
     d.npack = nid / 16;
@@ -252,107 +275,218 @@ int ds_set_genotypes(epi_ctx *c, Dataset &d, const int32_t *words, int nid, int 
     return EPI_OK;
 }
 
+// New genotypes. A failure after the arguments are checked leaves the
+// dataset empty (genotypes and phenotypes unset), never half updated.
+int ds_set_genotypes(epi_ctx *c, Dataset &d, const int32_t *words, int nid, int nsnp)
+{
+    if (!words || nid < 2 || nsnp < 2) return fail(c, EPI_ERR_ARG, "need nid >= 2, nsnp >= 2 and words");
+    if (nid > MAX_NID) return fail(c, EPI_ERR_RANGE, "nid %d above the supported %d", nid, MAX_NID);
+    d.marg_ok = false;
+    const int rc = ds_upload_genotypes(c, d, words, nid, nsnp);
+    if (rc) {
+        free_dataset(d);
+        d = Dataset();
+    }
+    return rc;
+}
+
+// Balanced signed 8-bit limbs of x, least significant first; false if x needs more than n.
+bool split_limbs(long long x, int n, int8_t *out)
+{
+    for (int i = 0; i < n; i++) {
+        const long long l = ((x % 256) + 256 + 128) % 256 - 128;
+        out[i] = (int8_t)l;
+        x = (x - l) / 256;
+    }
+    return x == 0;
+}
+
+// Host-side encoding of one phenotype, before anything is allocated.
+struct PhenEnc {
+    std::vector<int8_t> w;      // [NWP][Kp]
+    std::vector<int32_t> qv;    // [Kp]
+    std::vector<float> yc;      // [Kp]
+    std::vector<uint32_t> vb;   // [Kw]
+    std::vector<int32_t> oidx;  // outlier individuals
+    OutVals ov = {};
+    PhenDev p;                  // scalars only
+};
+
+// Phenotype prep (src/EpiOpenCL.c:1364-1376 computes mY and SSY over all nid
+// and the kernel drops -9 values, :135-147; here the missing values are
+// dropped up front). Centre, outliers and fixed-point scale of the MFMA
+// form, the plain form's scaled FP32 values, presence bits.
+int encode_phen(epi_ctx *c, const float *y, int nid, int Kp, bool nine_value, int r, PhenEnc &e)
+{
+    auto missing = [&](float v) { return v != v || (!nine_value && v == -9.0f); };
+    std::vector<int> P;
+    P.reserve(nid);
+    for (int k = 0; k < nid; k++) {
+        const float v = y[k];
+        if (missing(v)) continue;
+        if (std::isinf(v)) return fail(c, EPI_ERR_RANGE, "phenotype %d has an infinite value", r);
+        P.push_back(k);
+    }
+    const int nv = (int)P.size();
+    if (nv < 2) return fail(c, EPI_ERR_RANGE, "phenotype %d has fewer than 2 present values", r);
+    PhenDev &p = e.p;
+    // Outliers among the ntop values farthest from the mean, measured against
+    // the central values (all but the ncen farthest).
+    double c0 = 0.0;
+    for (int k : P) c0 += (double)y[k];
+    c0 /= nv;
+    const int ncen = std::min(2 * OUT_MAX, (nv - 2) / 4), ntop = std::min(OUT_MAX, ncen);
+    std::vector<int> ord(P);
+    auto farther = [&](int x, int z) {
+        const double dx = std::fabs((double)y[x] - c0), dz = std::fabs((double)y[z] - c0);
+        return dx != dz ? dx > dz : x < z;
+    };
+    if (ncen > 0) {
+        std::nth_element(ord.begin(), ord.begin() + ncen - 1, ord.end(), farther);
+        std::sort(ord.begin(), ord.begin() + ncen, farther);
+    }
+    std::vector<char> far(nid, 0), out(nid, 0);
+    for (int t = 0; t < ncen; t++) far[ord[t]] = 1;
+    double mb = 0.0, sb = 0.0;
+    for (int k : P)
+        if (!far[k]) mb += (double)y[k];
+    mb /= (nv - ncen);
+    for (int k : P)
+        if (!far[k]) sb += ((double)y[k] - mb) * ((double)y[k] - mb);
+    sb = std::sqrt(sb / (nv - ncen));
+    for (int t = 0; t < ntop; t++) {
+        const int k = ord[t];
+        if (std::fabs((double)y[k] - mb) > OUT_SD * sb) {
+            out[k] = 1;
+            e.oidx.push_back(k);
+        }
+    }
+    p.nout = (int)e.oidx.size();
+    double centre = 0.0;
+    for (int k : P)
+        if (!out[k]) centre += (double)y[k];
+    centre /= (nv - p.nout);
+    double maxabs = 0.0, maxall = 0.0;
+    for (int k : P) {
+        const double dv = std::fabs((double)y[k] - centre);
+        maxall = std::max(maxall, dv);
+        if (!out[k]) maxabs = std::max(maxabs, dv);
+    }
+    p.mfma_ok = maxabs <= BULK_SD * sb;
+    if (!p.mfma_ok) {
+        char buf[256];
+        snprintf(buf, sizeof(buf),
+                 "phenotype %d: more than %d values lie far out (after %d outliers, a value %.3g central SDs "
+                 "out remains, above %g); the MFMA form's fixed point cannot hold it to 1e-5, use the plain form",
+                 r, OUT_MAX, p.nout, sb > 0.0 ? maxabs / sb : INFINITY, BULK_SD);
+        p.mfma_why = buf;
+    }
+    int f = 0;
+    if (maxabs > 0.0) f = (int)std::floor(std::log2((double)QMAX / maxabs));
+    while (maxabs > 0.0 && std::ldexp(maxabs, f) > (double)QMAX) f--;
+    // the plain form's FP32 values are scaled by a power of two to |yc| < 1, so that products of
+    // phenotypes far from unit scale neither overflow nor underflow FP32 (F is scale-free)
+    int ye = 0;
+    if (maxall > 0.0) std::frexp(maxall, &ye);
+    const int Kw = Kp / 32;
+    e.w.assign((size_t)NWP * Kp, 0);
+    e.qv.assign(Kp, 0);
+    e.yc.assign(Kp, 0.0f);
+    e.vb.assign(Kw, 0u);
+    long long qmax2 = 0;
+    for (int k : P) {
+        const long long q = out[k] ? 0 : std::llround(std::ldexp((double)y[k] - centre, f));
+        if (q > QMAX || q < -QMAX) return fail(c, EPI_ERR_RANGE, "fixed-point phenotype out of range");
+        qmax2 = std::max(qmax2, q * q);
+        e.qv[k] = (int32_t)q;
+    }
+    // q^2 limbs: the smallest shift that fits four limbs (15 when |q| reaches QMAX)
+    int zb = 0;
+    while (std::llround(std::ldexp((double)qmax2, -zb)) > ZMAX) zb++;
+    long long qsum = 0;
+    double sq = 0.0, ys = 0.0, y2 = 0.0;
+    for (int k : P) {
+        const long long q = e.qv[k];
+        const long long z = std::llround(std::ldexp((double)(q * q), -zb));
+        int8_t lq[3], lz[4];
+        if (!split_limbs(q, 3, lq) || !split_limbs(z, 4, lz)) return fail(c, EPI_ERR_RANGE, "limb overflow");
+        e.w[0 * (size_t)Kp + k] = 1;
+        for (int t = 0; t < 3; t++) e.w[(1 + t) * (size_t)Kp + k] = lq[t];
+        for (int t = 0; t < 4; t++) e.w[(4 + t) * (size_t)Kp + k] = lz[t];
+        qsum += q;
+        sq += (double)q * (double)q;
+        const double dv = (double)y[k] - centre;
+        const float ycf = (float)std::ldexp(dv, -ye);
+        e.yc[k] = ycf;
+        ys += (double)ycf;
+        y2 += (double)ycf * (double)ycf;
+        e.vb[k >> 5] |= 1u << (k & 31);
+    }
+    for (int t = 0; t < p.nout; t++) {
+        const double u = std::ldexp((double)y[e.oidx[t]] - centre, f);
+        e.ov.d[t] = u;
+        e.ov.f[t] = (float)u;
+    }
+    p.fbits = f;
+    p.zbits = zb;
+    p.ybits = ye;
+    p.centre = centre;
+    p.nV = nv;
+    p.qV = qsum;
+    p.sqV = sq;
+    p.ysV = ys;
+    p.y2V = y2;
+    return EPI_OK;
+}
+
+// Encode every phenotype on the host, then allocate and upload. On any
+// failure the dataset is left without phenotypes (a scan then reports
+// EPI_ERR_STATE), never with a partly built set.
 int ds_set_phenotypes(epi_ctx *c, Dataset &d, const float *phen, int nphen, int nid, int flags)
 {
     const bool nine_value = (flags & EPI_PHEN_NINE_IS_VALUE) != 0;
-    auto missing = [&](float v) { return v != v || (!nine_value && v == -9.0f); };
     if (!phen || nphen < 1 || nid < 2) return fail(c, EPI_ERR_ARG, "need phenotypes");
     if (nid > MAX_NID) return fail(c, EPI_ERR_RANGE, "nid %d above the supported %d", nid, MAX_NID);
     if (d.nid && nid != d.nid) return fail(c, EPI_ERR_ARG, "nid %d differs from genotypes' nid %d", nid, d.nid);
+    drop_phenotypes(d);
     const int Kp = (nid + KPAD - 1) / KPAD * KPAD, Kw = Kp / 32;
-    for (auto &p : d.ph) free_phen(p);
-    d.ph.assign(nphen, PhenDev());
+    std::vector<PhenEnc> enc(nphen);
+    for (int r = 0; r < nphen; r++) {
+        int rc = encode_phen(c, phen + (size_t)r * nid, nid, Kp, nine_value, r, enc[r]);
+        if (rc) return rc;
+    }
+    std::vector<PhenDev> ph(nphen);
+    int rc = EPI_OK;
+    for (int r = 0; r < nphen && rc == EPI_OK; r++) {
+        PhenEnc &e = enc[r];
+        PhenDev &p = ph[r];
+        p = e.p;
+        if ((rc = dalloc(c, p.wts, e.w.size())) || (rc = dalloc(c, p.qv, (size_t)Kp)) ||
+            (rc = dalloc(c, p.yc, (size_t)Kp)) || (rc = dalloc(c, p.vbits, (size_t)Kw)) ||
+            (rc = dalloc(c, p.ov, 1)))
+            break;
+        if (p.nout && (rc = dalloc(c, p.oidx, (size_t)p.nout))) break;
+        hipError_t he = hipMemcpyAsync(p.wts, e.w.data(), e.w.size(), hipMemcpyHostToDevice, c->stream);
+        if (he == hipSuccess) he = hipMemcpyAsync(p.qv, e.qv.data(), (size_t)Kp * 4, hipMemcpyHostToDevice, c->stream);
+        if (he == hipSuccess) he = hipMemcpyAsync(p.yc, e.yc.data(), (size_t)Kp * 4, hipMemcpyHostToDevice, c->stream);
+        if (he == hipSuccess)
+            he = hipMemcpyAsync(p.vbits, e.vb.data(), (size_t)Kw * 4, hipMemcpyHostToDevice, c->stream);
+        if (he == hipSuccess) he = hipMemcpyAsync(p.ov, &e.ov, sizeof(OutVals), hipMemcpyHostToDevice, c->stream);
+        if (he == hipSuccess && p.nout)
+            he = hipMemcpyAsync(p.oidx, e.oidx.data(), (size_t)p.nout * 4, hipMemcpyHostToDevice, c->stream);
+        if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+        if (he != hipSuccess) rc = fail(c, EPI_ERR_HIP, "phenotype upload failed: %s", hipGetErrorString(he));
+    }
+    if (rc) {
+        for (auto &p : ph) free_phen(p);
+        return rc;
+    }
     if (!d.nid) {
         d.nid = nid;
         d.Kp = Kp;
         d.Kw = Kw;
     }
-    std::vector<int8_t> w((size_t)NWP * Kp);  // row NWP - 1 stays zero
-    std::vector<int32_t> qv(Kp);
-    std::vector<float> yc(Kp);
-    std::vector<uint32_t> vb(Kw);
-    for (int r = 0; r < nphen; r++) {
-        const float *y = phen + (size_t)r * nid;
-        PhenDev &p = d.ph[r];
-        double sum = 0.0;
-        int nv = 0;
-        for (int k = 0; k < nid; k++) {
-            float v = y[k];
-            if (missing(v)) continue;
-            if (std::isinf(v)) return fail(c, EPI_ERR_RANGE, "phenotype %d has an infinite value", r);
-            sum += v;
-            nv++;
-        }
-        if (nv < 2) return fail(c, EPI_ERR_RANGE, "phenotype %d has fewer than 2 present values", r);
-        double centre = sum / nv, maxabs = 0.0;
-        for (int k = 0; k < nid; k++) {
-            float v = y[k];
-            if (missing(v)) continue;
-            maxabs = std::max(maxabs, std::fabs((double)v - centre));
-        }
-        int f = 0;
-        if (maxabs > 0.0) f = (int)std::floor(std::log2((double)QMAX / maxabs));
-        while (maxabs > 0.0 && std::ldexp(maxabs, f) > (double)QMAX) f--;
-        // the plain form's FP32 values are scaled by a power of two to |yc| < 1, so that products of
-        // phenotypes far from unit scale neither overflow nor underflow FP32 (F is scale-free)
-        int e = 0;
-        if (maxabs > 0.0) std::frexp(maxabs, &e);
-        std::fill(w.begin(), w.end(), 0);
-        std::fill(qv.begin(), qv.end(), 0);
-        std::fill(yc.begin(), yc.end(), 0.0f);
-        std::fill(vb.begin(), vb.end(), 0u);
-        long long qsum = 0;
-        double sq = 0.0, ys = 0.0, y2 = 0.0;
-        for (int k = 0; k < nid; k++) {
-            float v = y[k];
-            if (missing(v)) continue;
-            double dv = (double)v - centre;
-            long long q = std::llround(std::ldexp(dv, f));
-            if (q > QMAX || q < -QMAX) return fail(c, EPI_ERR_RANGE, "fixed-point phenotype out of range");
-            long long l0 = ((q % 256) + 256 + 128) % 256 - 128;
-            long long q1 = (q - l0) / 256;
-            long long l1 = ((q1 % 256) + 256 + 128) % 256 - 128;
-            long long l2 = (q1 - l1) / 256;
-            long long z = std::llround((double)q * (double)q / ZSHIFT);
-            long long z0 = ((z % 256) + 256 + 128) % 256 - 128;
-            long long z1q = (z - z0) / 256;
-            long long z1 = ((z1q % 256) + 256 + 128) % 256 - 128;
-            long long z2 = (z1q - z1) / 256;
-            if (l2 < -128 || l2 > 127 || z2 < -128 || z2 > 127) return fail(c, EPI_ERR_RANGE, "limb overflow");
-            w[0 * (size_t)Kp + k] = 1;
-            w[1 * (size_t)Kp + k] = (int8_t)l0;
-            w[2 * (size_t)Kp + k] = (int8_t)l1;
-            w[3 * (size_t)Kp + k] = (int8_t)l2;
-            w[4 * (size_t)Kp + k] = (int8_t)z0;
-            w[5 * (size_t)Kp + k] = (int8_t)z1;
-            w[6 * (size_t)Kp + k] = (int8_t)z2;
-            qv[k] = (int32_t)q;
-            qsum += q;
-            sq += (double)q * (double)q;
-            float ycf = (float)std::ldexp(dv, -e);
-            yc[k] = ycf;
-            ys += (double)ycf;
-            y2 += (double)ycf * (double)ycf;
-            vb[k >> 5] |= 1u << (k & 31);
-        }
-        p.fbits = f;
-        p.ybits = e;
-        p.centre = centre;
-        p.nV = nv;
-        p.qV = qsum;
-        p.sqV = sq;
-        p.ysV = ys;
-        p.y2V = y2;
-        int rc;
-        if ((rc = dalloc(c, p.wts, w.size())) || (rc = dalloc(c, p.qv, (size_t)Kp)) ||
-            (rc = dalloc(c, p.yc, (size_t)Kp)) || (rc = dalloc(c, p.vbits, (size_t)Kw)))
-            return rc;
-        HIPCHK(c, hipMemcpyAsync(p.wts, w.data(), w.size(), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(p.qv, qv.data(), (size_t)Kp * 4, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(p.yc, yc.data(), (size_t)Kp * 4, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(p.vbits, vb.data(), (size_t)Kw * 4, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-    }
+    d.ph.swap(ph);
     d.marg_ok = false;
     return EPI_OK;
 }
@@ -376,10 +510,13 @@ int ds_set_chromosomes(epi_ctx *c, Dataset &d, const int32_t *chrstart, const in
     return EPI_OK;
 }
 
+// Per-SNP marginals and outlier codes of every phenotype (they depend on both
+// the genotypes and the phenotypes).
 int ds_marginals(epi_ctx *c, Dataset &d)
 {
     if (d.marg_ok) return EPI_OK;
-    if (!d.mbits || d.ph.empty()) return fail(c, EPI_ERR_STATE, "genotypes and phenotypes must be set");
+    if (!d.mbits) return fail(c, EPI_ERR_STATE, "genotypes not set");
+    if (d.ph.empty()) return fail(c, EPI_ERR_STATE, "phenotypes not set");
     for (auto &p : d.ph) {
         int rc = dalloc(c, p.marg, (size_t)d.nsnp);
         if (rc) return rc;
@@ -387,9 +524,22 @@ int ds_marginals(epi_ctx *c, Dataset &d)
                            (const uint32_t *)d.mbits, d.Kw, d.nsnp, (const uint32_t *)p.vbits, (const int32_t *)p.qv,
                            (const float *)p.yc, (const int32_t *)d.cls, p.marg);
         HIPCHK(c, hipGetLastError());
+        if (p.nout) {
+            if ((rc = dalloc(c, p.ocodes, (size_t)(p.nout + 15) / 16 * d.nsnp))) return rc;
+            hipLaunchKernelGGL(outlier_codes_kernel, dim3((d.nsnp + 255) / 256), dim3(256), 0, c->stream,
+                               (const uint32_t *)d.mbits, d.Kw, d.nsnp, (const int32_t *)p.oidx, p.nout, p.ocodes);
+            HIPCHK(c, hipGetLastError());
+        }
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     d.marg_ok = true;
+    return EPI_OK;
+}
+
+// MFMA scans of phenotype r need a phenotype the fixed point holds to 1e-5.
+int check_mfma_phen(epi_ctx *c, const Dataset &d, int r)
+{
+    if (r >= 0 && r < (int)d.ph.size() && !d.ph[r].mfma_ok) return fail(c, EPI_ERR_RANGE, "%s", d.ph[r].mfma_why.c_str());
     return EPI_OK;
 }
 
@@ -408,6 +558,10 @@ MfmaScan mfma_scan(const Dataset &d, int r, int tag)
     S.qV = p.qV;
     S.qVd = (double)p.qV;
     S.sqV = p.sqV;
+    S.zunit = std::ldexp(1.0, p.zbits);
+    S.ocodes = p.ocodes;
+    S.ov = p.ov;
+    S.nout = p.nout;
     return S;
 }
 
@@ -592,8 +746,10 @@ int check_params(epi_ctx *c, const epi_scan_params *p)
     int rc = check_common(c, p);
     if (rc) return rc;
     if (!c->d.mbits) return fail(c, EPI_ERR_STATE, "genotypes not set");
+    if (c->d.ph.empty()) return fail(c, EPI_ERR_STATE, "phenotypes not set");
     if (p->phen < 0 || p->phen >= (int)c->d.ph.size()) return fail(c, EPI_ERR_ARG, "phenotype index out of range");
     if (c->d.chrstart.empty()) return fail(c, EPI_ERR_STATE, "chromosomes not set");
+    if (p->form == EPI_FORM_MFMA) return check_mfma_phen(c, c->d, p->phen);
     return EPI_OK;
 }
 
@@ -860,6 +1016,9 @@ int capture_batch(epi_batch *b, const epi_scan_params *p)
 int batch_prepare(epi_batch *b, const epi_scan_params *p)
 {
     int rc;
+    if (p->form == EPI_FORM_MFMA)
+        for (const auto &x : b->scans)
+            if ((rc = check_mfma_phen(b->ctx, *b->data[x.first], x.second))) return rc;
     const epi_schedule &sc = b->sched[p->form];
     if (b->dirty || b->work_form != p->form || !same_schedule(b->work_sched, sc)) {
         drop_graph(b);
@@ -1411,7 +1570,10 @@ int epi_batch_add_phenotypes_ex(epi_batch *b, int32_t dataset, const float *phen
     HIPCHK(c, hipSetDevice(c->device));
     int rc = ds_set_phenotypes(c, d, phen, nphen, d.nid, flags);
     if (!rc) rc = ds_marginals(c, d);
-    if (rc) return rc;
+    if (rc) {
+        drop_phenotypes(d);
+        return rc;
+    }
     if (first_scan) *first_scan = (int32_t)b->scans.size();
     for (int r = 0; r < nphen; r++) b->scans.push_back({dataset, r});
     b->dirty = true;

@@ -6,9 +6,15 @@
 // where I_g are 0/1 class indicators (g in {0, 1, NM}) and W_w the per-
 // individual weights: W_0 = V (phenotype present, 0/1), W_1..3 = the three
 // signed 8-bit limbs of the fixed-point phenotype q = L0 + 256 L1 + 65536 L2,
-// W_4..6 the limbs of round(q^2 / 2^23). Every product and every INT32 sum is
-// exact, so counts are bit-exact and cell sums are exact integers in units of
-// the fixed-point phenotype.
+// W_4..7 the four limbs of round(q^2 / 2^zbits). Every product and every INT32
+// sum is exact, so counts are bit-exact and cell sums are exact integers in
+// units of the fixed-point phenotype.
+//
+// Outliers: the fixed-point scale is set by the largest encoded value, so a
+// few values far out in the tail would coarsen every other individual. Up to
+// OUT_MAX such values per phenotype are carried exactly instead: in the
+// contraction they have V = 1 (they count) and zero value limbs, and their
+// values are added to each pair's table in the epilogue (add_outliers).
 //
 // Operands: class masks are stored as bitplanes in HBM ([plane][word][snp],
 // bit k of word w = individual 32w + k, planes: class 0, class 1, call
@@ -36,8 +42,8 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 
 constexpr int KC = 128;          // individuals per LDS stage
 constexpr int ROWB = KC + 16;    // LDS row pitch in bytes (bank-conflict padding)
-constexpr int NW = 7;            // weight rows: V, L0, L1, L2, Z0, Z1, Z2
-constexpr int NWP = 8;           // rows in memory: the NW rows and an all-zero row
+constexpr int NWP = 8;           // weight rows: V, L0, L1, L2, Z0, Z1, Z2, Z3
+constexpr int OUT_MAX = 32;      // outlier individuals carried exactly, per phenotype
 
 // Identity class map: plane p holds class p (the MFMA form keeps genotype order).
 constexpr int CLS_IDENTITY = 0 | (1 << 2) | (2 << 4);
@@ -51,19 +57,62 @@ struct Marg {                    // per SNP, per phenotype, over present phenoty
     int32_t pad;
 };
 
+struct OutVals {                 // values of the outlier individuals, in fixed-point units
+    double d[OUT_MAX];
+    float f[OUT_MAX];
+};
+
 struct MfmaScan {                // one dataset x one phenotype
     const uint32_t *bits;        // [3][Kw][nsnp] bitplanes: class 0, class 1, call present
     const int8_t *wts;           // [NWP][Kp] for this phenotype
     const Marg *marg;            // [nsnp] for this phenotype
+    const uint32_t *ocodes;      // [ceil(nout/16)][nsnp] 2-bit codes of the outliers (class, 3 = no call)
+    const OutVals *ov;           // outlier values
+    int nout;                    // outliers (0..OUT_MAX)
     int nsnp;
     int Kp;                      // = 32 Kw: row stride of wts, plane stride of bits (32 Kp / 32 dwords)
     int Kl;                      // individuals the K loop covers: nid rounded up to KC (<= Kp)
     int nV;                      // present phenotypes
     int tag;                     // written into the hits
-    long long qV;                // sum q over present
+    long long qV;                // sum q over present (outliers excluded)
     double qVd;                  // the same, as a double (exact)
-    double sqV;                  // sum q^2 over present
+    double sqV;                  // sum q^2 over present (outliers excluded)
+    double zunit;                // 2^zbits: q^2 units per unit of the Z limbs
 };
+
+// Adds the outlier individuals that are complete cases of pair (a, b) to its
+// {0,1,NM}^2 table x of sums and to the sum of squares sq (fixed-point units).
+// Word w of a SNP's codes holds outliers 16w..16w+15, two bits each: the
+// genotype class, or 3 for a missing call. The loop bound is wave-uniform.
+template <class T>
+__device__ __forceinline__ void add_outliers(const MfmaScan &S, int a, int b, T x[3][3], T &sq)
+{
+    const int an = a < S.nsnp ? a : S.nsnp - 1, bn = b < S.nsnp ? b : S.nsnp - 1;
+    for (int w = 0; 16 * w < S.nout; w++) {
+        const uint32_t ca = S.ocodes[(size_t)w * S.nsnp + an], cb = S.ocodes[(size_t)w * S.nsnp + bn];
+        const int nt = S.nout - 16 * w < 16 ? S.nout - 16 * w : 16;
+        for (int t = 0; t < nt; t++) {
+            const uint32_t ga = (ca >> (2 * t)) & 3u, gb = (cb >> (2 * t)) & 3u;
+            T v;
+            if constexpr (sizeof(T) == 8)
+                v = S.ov->d[16 * w + t];
+            else
+                v = S.ov->f[16 * w + t];
+            const T u = (ga != 3u && gb != 3u) ? v : (T)0;
+            const T u0 = ga == 0u ? u : (T)0, u1 = ga == 1u ? u : (T)0;
+            x[0][0] += gb == 0u ? u0 : (T)0;
+            x[0][1] += gb == 1u ? u0 : (T)0;
+            x[1][0] += gb == 0u ? u1 : (T)0;
+            x[1][1] += gb == 1u ? u1 : (T)0;
+            x[0][2] += u0;
+            x[1][2] += u1;
+            x[2][0] += gb == 0u ? u : (T)0;
+            x[2][1] += gb == 1u ? u : (T)0;
+            x[2][2] += u;
+            sq += u * u;
+        }
+    }
+}
 
 struct MfmaArgs {
     MfmaScan S;
@@ -412,7 +461,7 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
         }
     }
     __syncthreads();
-    const float sqf = (float)A.sqV, qVf = (float)A.qV;
+    const float sqV = (float)A.sqV, qVf = (float)A.qV;
 
     // Epilogue: lane holds pairs (a = a0 + APW w + 4i + 2hf + p, b = b0 + 32bg + m);
     // acc[i][2bg+h][8p + 4g + w] = X[g][h][w].
@@ -444,6 +493,8 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
                 }
                 xc[2][2] = A.nV;
                 xf[2][2] = qVf;
+                float sqf = sqV;
+                if (A.nout) add_outliers(A, a, b, xf, sqf);
                 int n[9];
                 float sf[9];
                 cells_from_x(xc, n);
@@ -463,8 +514,9 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
                         xs[2][g] = mg_q[2 * bl + g];
                     }
                     xs[2][2] = A.qVd;
-                    cells_from_x(xs, sd);
                     sq = A.sqV;
+                    if (A.nout) add_outliers(A, a, b, xs, sq);
+                    cells_from_x(xs, sd);
                 };
                 const bool valid = a < a1 && b < b1 && a > b;
                 finish_pair<DUMP>(hs, dsk, valid, a, b, n, sf, sqf, exact, CLS_IDENTITY, CLS_IDENTITY);
@@ -493,10 +545,10 @@ __global__ __launch_bounds__(fast_threads<TA>(), 64 / TA) void mfma_fast_kernel(
 // wave's eight with weight m & 3 (V, L0, L1, L2), so that lane half hf of the
 // 32x32 accumulator holds a-SNPs 4hf..4hf+3 with all four weights. Each A
 // block meets the three B classes (0, 1, NM): 9 MFMAs. A fourth block, the
-// same NM masks with weights Z0, Z1, Z2 and the zero row, meets only the NM
-// class of B for the complete-case sum of q^2: 10 MFMAs per 32-individual
-// step for 8 x 32 pairs (40 MACs per pair and individual, 39 of them needed)
-// and 160 accumulator registers, two waves per SIMD.
+// same NM masks with weights Z0..Z3 (the limbs of q^2 / 2^zbits), meets only
+// the NM class of B for the complete-case sum of q^2: 10 MFMAs per
+// 32-individual step for 8 x 32 pairs (40 MACs per pair and individual) and
+// 160 accumulator registers, two waves per SIMD.
 template <int GA>
 constexpr int general_lds() { return 2 * (3 * (GA + 32) + NWP + 1) * ROWB; }
 
@@ -511,14 +563,14 @@ __device__ __forceinline__ void mfma_general_tile(const MfmaScan &A, const HitSi
     constexpr int TA = GA, TB = 32, NPL = 3, NT = general_threads<GA>();
     constexpr int ROWS_AB = NPL * (TA + TB);
     constexpr int NCH = (ROWS_AB * (KC / 32) + NT - 1) / NT;  // bitplane dwords per thread
-    constexpr int STAGE = (ROWS_AB + NWP + 1) * ROWB;          // + weight rows (the last all zero), a dummy row
+    constexpr int STAGE = (ROWS_AB + NWP + 1) * ROWB;          // + weight rows, a dummy row
     static_assert(2 * STAGE == general_lds<GA>(), "LDS size");
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int Kp = A.Kp;
 
     // Global -> LDS: ROWS_AB rows of bitplanes, expanded to bytes, and the NWP
-    // weight rows V, L0, L1, L2, Z0, Z1, Z2 and the all-zero row.
+    // weight rows V, L0, L1, L2, Z0, Z1, Z2, Z3.
     Stager<NPL, TA, TB, NCH, NWP, NT> sg(A.bits, A.wts, Kp, A.nsnp, a0, b0, tid);
 
     const int m = lane & 31, hf = lane >> 5;
@@ -526,7 +578,7 @@ __device__ __forceinline__ void mfma_general_tile(const MfmaScan &A, const HitSi
     const int wr = m & 3;                          // weight of A row m
     const int offA = (8 * wid + aj) * ROWB + 16 * hf;                           // + g * TA * ROWB
     const int offW = (ROWS_AB + wr) * ROWB + 16 * hf;                           // V, L0, L1, L2
-    const int offZ = (ROWS_AB + (wr < 3 ? 4 + wr : NW)) * ROWB + 16 * hf;       // Z0, Z1, Z2, zero row
+    const int offZ = (ROWS_AB + 4 + wr) * ROWB + 16 * hf;                       // Z0, Z1, Z2, Z3
     const int offB = (NPL * TA + m) * ROWB + 16 * hf;                           // + h * TB * ROWB
 
     v16i acc[3][3], accz = (v16i){};
@@ -580,8 +632,10 @@ __device__ __forceinline__ void mfma_general_tile(const MfmaScan &A, const HitSi
                 xc[g][h] = v[4 * j];
                 xf[g][h] = (float)v[4 * j + 1] + 256.0f * (float)v[4 * j + 2] + 65536.0f * (float)v[4 * j + 3];
             }
-        const float sqf =
-            ((float)accz[4 * j] + 256.0f * (float)accz[4 * j + 1] + 65536.0f * (float)accz[4 * j + 2]) * 8388608.0f;
+        float sqf = ((float)accz[4 * j] + 256.0f * (float)accz[4 * j + 1] + 65536.0f * (float)accz[4 * j + 2] +
+                     16777216.0f * (float)accz[4 * j + 3]) *
+                    (float)A.zunit;
+        if (A.nout) add_outliers(A, a, b, xf, sqf);
         int n[9];
         float sf[9];
         cells_from_x(xc, n);
@@ -596,9 +650,11 @@ __device__ __forceinline__ void mfma_general_tile(const MfmaScan &A, const HitSi
                     const v16i &v = acc[g][h];
                     xs[g][h] = (double)v[4 * j + 1] + 256.0 * (double)v[4 * j + 2] + 65536.0 * (double)v[4 * j + 3];
                 }
+            sq = ((double)accz[4 * j] + 256.0 * (double)accz[4 * j + 1] + 65536.0 * (double)accz[4 * j + 2] +
+                  16777216.0 * (double)accz[4 * j + 3]) *
+                 A.zunit;
+            if (A.nout) add_outliers(A, a, b, xs, sq);
             cells_from_x(xs, sd);
-            sq = ((double)accz[4 * j] + 256.0 * (double)accz[4 * j + 1] + 65536.0 * (double)accz[4 * j + 2]) *
-                 8388608.0;
         };
         const bool valid = a < a1 && b < b1 && a > b;
         finish_pair<DUMP>(hs, dsk, valid, a, b, n, sf, sqf, exact, CLS_IDENTITY, CLS_IDENTITY);

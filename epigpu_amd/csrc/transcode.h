@@ -137,4 +137,28 @@ __global__ __launch_bounds__(256) void marginals_kernel(const uint32_t *mbits, i
     }
 }
 
+// Codes of one phenotype's outlier individuals per SNP (thread = SNP): two
+// bits per outlier, the genotype class (0, 1, 2) or 3 for a missing call;
+// word w holds outliers 16w..16w+15, stored [w][snp] so that a wave's SNPs
+// read consecutive words.
+__global__ __launch_bounds__(256) void outlier_codes_kernel(const uint32_t *mbits, int Kw, int nsnp,
+                                                            const int32_t *oidx, int nout, uint32_t *ocodes)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsnp) return;
+    for (int w = 0; 16 * w < nout; w++) {
+        uint32_t code = 0;
+        for (int t = 0; t < 16 && 16 * w + t < nout; t++) {
+            const int k = oidx[16 * w + t], wd = k >> 5;
+            const uint32_t bit = 1u << (k & 31);
+            const bool c0 = (mbits[((size_t)0 * Kw + wd) * nsnp + s] & bit) != 0;
+            const bool c1 = (mbits[((size_t)1 * Kw + wd) * nsnp + s] & bit) != 0;
+            const bool nm = (mbits[((size_t)2 * Kw + wd) * nsnp + s] & bit) != 0;
+            const uint32_t g = !nm ? 3u : c0 ? 0u : c1 ? 1u : 2u;
+            code |= g << (2 * t);
+        }
+        ocodes[(size_t)w * nsnp + s] = code;
+    }
+}
+
 }  // namespace epi

@@ -31,8 +31,10 @@ int perform_scan(int *geno, chromosome *chrstat, ped *dat, map *genmap, int nid,
     epi_run_opts o;
     epi_run_opts_default(&o);
     o.test = UTEST == 'i' ? 'i' : 'f';
-    if (UTHRESHOLD1 != 0.0f) o.t1 = UTHRESHOLD1;
-    if (UTHRESHOLD2 != 0.0f) o.t2 = UTHRESHOLD2;
+    /* used as given, 0 included: the reference's main() sets 6.5 / 10.5 before parsing -F / -I
+       (src/main.c:247-249) and its kernels take the globals unchanged (src/EpiOpenCL.c:1516-1517) */
+    o.t1 = UTHRESHOLD1;
+    o.t2 = UTHRESHOLD2;
     o.device = UDEVICE;
     o.perm = UPERM;
     o.safe = USAFE ? 1 : 0;

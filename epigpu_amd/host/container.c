@@ -76,6 +76,14 @@ int epi_read_container(const char *path, epi_container *c)
              fread(c->chrstat, sizeof(chromosome), (size_t)c->nchr, f) == (size_t)c->nchr &&
              fread(c->geno, sizeof(int32_t), npack1 * (size_t)c->nsnp, f) == npack1 * (size_t)c->nsnp;
     fclose(f);
+    /* Name fields are printed with %s: each must end within its 30 bytes. */
+    for (int i = 0; ok && i < c->nid; i++)
+        ok = memchr(c->dat[i].family, 0, 30) && memchr(c->dat[i].id, 0, 30) && memchr(c->dat[i].paternal, 0, 30) &&
+             memchr(c->dat[i].maternal, 0, 30);
+    for (int i = 0; ok && i < c->nsnp; i++) ok = memchr(c->genmap[i].chr, 0, 30) && memchr(c->genmap[i].snpname, 0, 30);
+    for (int i = 0; ok && i < c->nchr; i++)
+        ok = memchr(c->chrstat[i].chrname, 0, 30) && c->chrstat[i].chrsize >= 0 && c->chrstat[i].chrstart >= 0 &&
+             (long long)c->chrstat[i].chrstart + c->chrstat[i].chrsize <= c->nsnp;
     if (!ok) {
         epi_container_free(c);
         return EPIH_ERR_FORMAT;

@@ -8,10 +8,14 @@
  *     -f p       set -F from a p-value: F^-1(1-p; 8, N-9) (extension)
  *     -g p       set -I from a p-value: F^-1(1-p; 4, N-9) (extension)
  *     -d n       HIP device (default 0)
- *     -p n       permutation n: shuffle phenotypes with srand(n), as the reference
+ *     -p n       permutation n: shuffle phenotypes with srand(n) and write perm<n>.txt,
+ *                as the reference (src/main.c:309-318, :336-357)
  *     -p a-b     permutations a..b in one batched device launch, one output file
- *                <output.txt>.perm<k> per permutation (extension)
- *     -P file    phenotype file, one value per individual
+ *                <output.txt>.perm<k> and one perm<k>.txt per permutation (extension)
+ *     -P file    phenotype file, one value per individual (src/main.c:172-181, :191-222).
+ *                Repeatable (extension): several files scan in one batched device
+ *                launch, file j (1-based) into <output.txt>.P<j>; with -p, every file's
+ *                permutations too, into <output.txt>.P<j>.perm<k> and perm<k>.P<j>.txt
  *     -m plain|mfma  kernel form (default mfma) (extension)
  *     -s 0|1     safe mode: -9 is a phenotype value and test i drops nfac > 6, as
  *                the reference's safe kernels (src/EpiOpenCL.c:505-963)
@@ -40,7 +44,7 @@ static void usage(void)
 {
     fprintf(stderr,
             "usage: episcan -A <data.epigpu> <output.txt> [-t f|i] [-F x] [-I x] [-f p] [-g p]\n"
-            "               [-d n] [-p n | -p a-b] [-P file] [-m plain|mfma] [-s 0|1]\n"
+            "               [-d n] [-p n | -p a-b] [-P file ...] [-m plain|mfma] [-s 0|1]\n"
             "       episcan -Dr[c][m][n] <file.ped> <file.map> <out.epigpu>\n"
             "       episcan -Db[c][m][n] <prefix> <out.epigpu>\n"
             "       episcan -De <file.ped> <file.map> <in.epigpu>\n");
@@ -167,7 +171,9 @@ int main(int argc, char **argv)
     const char *bin = argv[2], *outfile = argv[3];
     epi_run_opts o;
     epi_run_opts_default(&o);
-    const char *phenfile = "";
+    enum { MAXP = 256 };
+    const char *phenfiles[MAXP];
+    int nphenf = 0;
     double pf = -1.0, pi = -1.0;
     int pfirst = 0, plast = 0;
     for (int i = 4; i < argc; i += 2) {
@@ -201,7 +207,10 @@ int main(int argc, char **argv)
             }
             break;
         }
-        case 'P': phenfile = v; break;
+        case 'P':
+            if (nphenf == MAXP) { fprintf(stderr, "at most %d phenotype files\n", MAXP); return 1; }
+            phenfiles[nphenf++] = v;
+            break;
         case 'm':
             if (strcmp(v, "plain") == 0) o.form = EPI_FORM_PLAIN;
             else if (strcmp(v, "mfma") == 0) o.form = EPI_FORM_MFMA;
@@ -222,44 +231,96 @@ int main(int argc, char **argv)
         fprintf(stderr, "%s: %s\n", bin, epih_strerror(rc));
         return 1;
     }
-    if (phenfile[0]) {
-        FILE *pfh = fopen(phenfile, "rt");
-        if (!pfh) { fprintf(stderr, "Separate phenotype file does not exist\n"); return 1; }
+    /* phenotype sources: the container's phenotypes, or one vector per -P file */
+    const int nsrc = nphenf ? nphenf : 1;
+    float *src = (float *)malloc(sizeof(float) * (size_t)c.nid * (size_t)nsrc);
+    int32_t *order = (int32_t *)malloc(sizeof(int32_t) * (size_t)c.nid);
+    if (!src || !order) { fprintf(stderr, "out of memory\n"); return 1; }
+    for (int i = 0; i < c.nid; i++) src[i] = c.dat[i].phen;
+    for (int jf = 0; jf < nphenf; jf++) {
+        FILE *pfh = fopen(phenfiles[jf], "rt");
+        if (!pfh) { fprintf(stderr, "Separate phenotype file does not exist: %s\n", phenfiles[jf]); return 1; }
         for (int i = 0; i < c.nid; i++) {
             float t;
-            if (fscanf(pfh, "%f", &t) != 1) { fprintf(stderr, "phenotype file too short\n"); return 1; }
-            c.dat[i].phen = t;
+            if (fscanf(pfh, "%f", &t) != 1) { fprintf(stderr, "phenotype file too short: %s\n", phenfiles[jf]); return 1; }
+            src[(size_t)jf * c.nid + i] = t;
         }
         fclose(pfh);
     }
-    if (is_binary(c.nid, c.dat)) {
-        fprintf(stderr, "Error: Binary phenotype!\nepiGPU can only use quantitative phenotypes for analysis.\n");
-        return 1;
-    }
-    if (o.perm > 0 && pfirst == 0) {
-        float *ph = (float *)malloc(sizeof(float) * (size_t)c.nid);
-        if (!ph) { fprintf(stderr, "out of memory\n"); return 1; }
-        epi_permute_phen((unsigned)o.perm, c.nid, c.dat, ph);
-        for (int i = 0; i < c.nid; i++) c.dat[i].phen = ph[i];
-        free(ph);
+    for (int jf = 0; jf < nsrc; jf++) {
+        for (int i = 0; i < c.nid; i++) c.dat[i].phen = src[(size_t)jf * c.nid + i];
+        if (is_binary(c.nid, c.dat)) {
+            fprintf(stderr, "Error: Binary phenotype!\nepiGPU can only use quantitative phenotypes for analysis.\n");
+            return 1;
+        }
     }
     if (pf > 0.0) o.t1 = (float)epi_f_isf(pf, 8.0, (double)c.nid - 9.0);
     if (pi > 0.0) o.t2 = (float)epi_f_isf(pi, 4.0, (double)c.nid - 9.0);
     o.datafile = base_name(bin);
-    o.phenfile = phenfile;
+    o.phenfile = nphenf ? phenfiles[0] : "";
     epi_run_report rep;
-    if (pfirst > 0) {
-        char fmt[4096];
-        if (strchr(outfile, '%') || snprintf(fmt, sizeof fmt, "%s.perm%%d", outfile) >= (int)sizeof fmt) {
-            fprintf(stderr, "output name unusable for -p a-b\n");
-            return 1;
+    if (nsrc == 1 && pfirst == 0) {
+        /* one scan, resumable: the reference's path, with its permutation (src/main.c:309-318) */
+        for (int i = 0; i < c.nid; i++) c.dat[i].phen = src[i];
+        if (o.perm > 0) {
+            char pname[64];
+            epi_permute_order((unsigned)o.perm, c.nid, order);
+            snprintf(pname, sizeof pname, "perm%d.txt", o.perm);
+            if (epi_write_perm_file(pname, c.nid, c.dat, order, src)) {
+                fprintf(stderr, "cannot write %s\n", pname);
+                return 1;
+            }
+            for (int i = 0; i < c.nid; i++) c.dat[i].phen = src[order[i]];
+            printf("Permutation %d\n", o.perm);
+            for (int i = 0; i < 10 && i < c.nid; i++) printf("%f\n", c.dat[i].phen);
         }
-        rc = epi_perform_scan_perms(&o, c.geno, c.chrstat, c.dat, c.genmap, c.nid, c.nsnp, c.npack, c.remain, c.nchr,
-                                    pfirst, plast, fmt, &rep);
-    } else {
         rc = epi_perform_scan(&o, c.geno, c.chrstat, c.dat, c.genmap, c.nid, c.nsnp, c.npack, c.remain, c.nchr,
                               outfile, &rep);
+    } else {
+        /* every (source, permutation) pair is one scan of a single batched launch */
+        if (pfirst == 0 && o.perm > 0) pfirst = plast = o.perm;
+        const int nperm = pfirst ? plast - pfirst + 1 : 1, nscan = nsrc * nperm;
+        float *ph = (float *)malloc(sizeof(float) * (size_t)c.nid * (size_t)nscan);
+        int *perms = (int *)malloc(sizeof(int) * (size_t)nscan);
+        const char **pfs = (const char **)malloc(sizeof(char *) * (size_t)nscan);
+        char **names = (char **)calloc((size_t)nscan, sizeof(char *));
+        const size_t nlen = strlen(outfile) + 64;
+        if (!ph || !perms || !pfs || !names) { fprintf(stderr, "out of memory\n"); return 1; }
+        for (int jf = 0; jf < nsrc; jf++)
+            for (int kk = 0; kk < nperm; kk++) {
+                const int s = jf * nperm + kk, k = pfirst ? pfirst + kk : 0;
+                const float *y = src + (size_t)jf * c.nid;
+                float *dst = ph + (size_t)s * c.nid;
+                char tag[32] = "", pname[96];
+                if (nsrc > 1) snprintf(tag, sizeof tag, ".P%d", jf + 1);
+                if (k) {
+                    epi_permute_order((unsigned)k, c.nid, order);
+                    for (int i = 0; i < c.nid; i++) dst[i] = y[order[i]];
+                    snprintf(pname, sizeof pname, "perm%d%s.txt", k, tag);
+                    if (epi_write_perm_file(pname, c.nid, c.dat, order, y)) {
+                        fprintf(stderr, "cannot write %s\n", pname);
+                        return 1;
+                    }
+                } else {
+                    memcpy(dst, y, sizeof(float) * (size_t)c.nid);
+                }
+                names[s] = (char *)malloc(nlen);
+                if (!names[s]) { fprintf(stderr, "out of memory\n"); return 1; }
+                if (k) snprintf(names[s], nlen, "%s%s.perm%d", outfile, tag, k);
+                else snprintf(names[s], nlen, "%s%s", outfile, tag);
+                perms[s] = k;
+                pfs[s] = nphenf ? phenfiles[jf] : "";
+            }
+        rc = epi_perform_scan_multi(&o, c.geno, c.chrstat, c.genmap, c.nid, c.nsnp, c.npack, c.remain, c.nchr, nscan,
+                                    ph, perms, pfs, (const char *const *)names, &rep);
+        for (int s = 0; s < nscan; s++) free(names[s]);
+        free(names);
+        free(pfs);
+        free(perms);
+        free(ph);
     }
+    free(order);
+    free(src);
     epi_container_free(&c);
     if (rc) {
         fprintf(stderr, "episcan: %s (%d)\n", epih_strerror(rc), rc);

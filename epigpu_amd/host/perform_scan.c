@@ -298,16 +298,40 @@ done:
     return rc;
 }
 
-void epi_permute_phen(unsigned seed, int nid, const ped *dat, float *phen_out)
+void epi_permute_order(unsigned seed, int nid, int32_t *order)
 {
-    for (int i = 0; i < nid; i++) phen_out[i] = dat[i].phen;
+    for (int i = 0; i < nid; i++) order[i] = i;
     srand(seed);
     for (int i = nid; i > 1; i--) {
         int r = rand() % i;
-        float t = phen_out[r];
-        phen_out[r] = phen_out[i - 1];
-        phen_out[i - 1] = t;
+        int32_t t = order[r];
+        order[r] = order[i - 1];
+        order[i - 1] = t;
     }
+}
+
+void epi_permute_phen(unsigned seed, int nid, const ped *dat, float *phen_out)
+{
+    int32_t *order = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nid > 0 ? nid : 1));
+    if (!order) {
+        for (int i = 0; i < nid; i++) phen_out[i] = dat[i].phen;
+        return;
+    }
+    epi_permute_order(seed, nid, order);
+    for (int i = 0; i < nid; i++) phen_out[i] = dat[order[i]].phen;
+    free(order);
+}
+
+int epi_write_perm_file(const char *path, int nid, const ped *dat, const int32_t *order, const float *phen)
+{
+    FILE *f = fopen(path, "w");
+    if (!f) return EPIH_ERR_IO;
+    fprintf(f, "Family ID Phenotype\n");
+    for (int i = 0; i < nid; i++) {
+        const int k = order[i];
+        fprintf(f, "%s %s %f\n", dat[k].family, dat[k].id, phen ? phen[k] : dat[k].phen);
+    }
+    return fclose(f) == 0 ? EPIH_OK : EPIH_ERR_IO;
 }
 
 /* Output file name of permutation k from a format with one %d. */
@@ -319,30 +343,29 @@ static int perm_name(char *buf, size_t n, const char *fmt, int k)
     return m > 0 && (size_t)m < n ? EPIH_OK : EPIH_ERR_ARG;
 }
 
-int epi_perform_scan_perms(const epi_run_opts *opt, const int32_t *geno, const chromosome *chrstat, const ped *dat,
-                           const map *genmap, int nid, int nsnp, int npack, int remain, int nchr, int first, int last,
-                           const char *fmt, epi_run_report *rep)
+int epi_perform_scan_multi(const epi_run_opts *opt, const int32_t *geno, const chromosome *chrstat,
+                           const map *genmap, int nid, int nsnp, int npack, int remain, int nchr, int nscan,
+                           const float *phen, const int *perms, const char *const *phenfiles,
+                           const char *const *outfiles, epi_run_report *rep)
 {
     double t0 = now_s();
     epi_run_report local;
     if (!rep) rep = &local;
     memset(rep, 0, sizeof(*rep));
-    if (!opt || !geno || !chrstat || !dat || !genmap || !fmt || nid < 2 || nsnp < 2 || nchr < 1 || first < 1 ||
-        last < first)
+    if (!opt || !geno || !chrstat || !genmap || !phen || !outfiles || nid < 2 || nsnp < 2 || nchr < 1 || nscan < 1)
         return EPIH_ERR_ARG;
     if (npack != nid / 16 || remain != nid % 16) return EPIH_ERR_ARG;
     if (opt->test != 'f' && opt->test != 'i') return EPIH_ERR_ARG;
-    const int np = last - first + 1;
-    char name[4096];
-    for (int k = first; k <= last; k++) {
-        if (perm_name(name, sizeof name, fmt, k)) return EPIH_ERR_ARG;
-        FILE *f = fopen(name, "r");
+    for (int s = 0; s < nscan; s++) {
+        if (!outfiles[s]) return EPIH_ERR_ARG;
+        for (int t = 0; t < s; t++)
+            if (strcmp(outfiles[s], outfiles[t]) == 0) return EPIH_ERR_ARG;
+        FILE *f = fopen(outfiles[s], "r");
         if (f) {
             fclose(f);
             return EPIH_ERR_DONE;
         }
     }
-    float *phen = (float *)malloc(sizeof(float) * (size_t)nid * (size_t)np);
     int32_t *cs = (int32_t *)malloc(sizeof(int32_t) * (size_t)nchr);
     int32_t *cz = (int32_t *)malloc(sizeof(int32_t) * (size_t)nchr);
     epi_ctx *ctx = NULL;
@@ -351,8 +374,7 @@ int epi_perform_scan_perms(const epi_run_opts *opt, const int32_t *geno, const c
     int32_t *blk = NULL;
     FILE *out = NULL;
     int rc = EPIH_OK;
-    if (!phen || !cs || !cz) { rc = EPIH_ERR_IO; goto done; }
-    for (int k = first; k <= last; k++) epi_permute_phen((unsigned)k, nid, dat, phen + (size_t)(k - first) * nid);
+    if (!cs || !cz) { rc = EPIH_ERR_IO; goto done; }
     for (int i = 0; i < nchr; i++) {
         cs[i] = chrstat[i].chrstart;
         cz[i] = chrstat[i].chrsize;
@@ -360,7 +382,7 @@ int epi_perform_scan_perms(const epi_run_opts *opt, const int32_t *geno, const c
     if ((rc = epi_open(opt->device, &ctx)) || (rc = epi_batch_create(ctx, &b))) goto done;
     int32_t d = 0, s0 = 0;
     if ((rc = epi_batch_add_dataset(b, geno, nid, nsnp, cs, cz, nchr, &d)) ||
-        (rc = epi_batch_add_phenotypes_ex(b, d, phen, np, opt->safe ? EPI_PHEN_NINE_IS_VALUE : 0, &s0))) {
+        (rc = epi_batch_add_phenotypes_ex(b, d, phen, nscan, opt->safe ? EPI_PHEN_NINE_IS_VALUE : 0, &s0))) {
         if (!opt->quiet) fprintf(stderr, "episcan: %s\n", epi_last_error(ctx));
         goto done;
     }
@@ -372,10 +394,13 @@ int epi_perform_scan_perms(const epi_run_opts *opt, const int32_t *geno, const c
     p.t2 = opt->t2;
     p.safe = opt->safe;
     if (!opt->quiet) {
-        printf("Scanning permutations %d..%d in one batched launch\n", first, last);
+        printf("Scanning %d phenotypes in one batched launch\n", nscan);
         fflush(stdout);
     }
-    if ((rc = epi_batch_scan(b, &p))) goto done;
+    if ((rc = epi_batch_scan(b, &p))) {
+        if (!opt->quiet) fprintf(stderr, "episcan: %s\n", epi_last_error(ctx));
+        goto done;
+    }
     int64_t tot = 0;
     int32_t ovf = 0;
     epi_batch_hit_count(b, -1, &tot, &ovf);
@@ -394,8 +419,8 @@ int epi_perform_scan_perms(const epi_run_opts *opt, const int32_t *geno, const c
         rep->pairs = pairs;
     }
     const int nblk = nchr * (nchr + 1) / 2;
-    for (int k = first; k <= last; k++) {
-        const int s = s0 + (k - first);
+    for (int sc = 0; sc < nscan; sc++) {
+        const int s = s0 + sc;
         int64_t nh = 0, got = 0;
         if ((rc = epi_batch_hit_count(b, s, &nh, NULL))) goto done;
         free(hits);
@@ -405,10 +430,10 @@ int epi_perform_scan_perms(const epi_run_opts *opt, const int32_t *geno, const c
         if (!hits || !blk) { rc = EPIH_ERR_IO; goto done; }
         if ((rc = epi_batch_get_hits(b, s, hits, NULL, NULL, blk, nh, &got))) goto done;
         epi_run_opts o = *opt;
-        o.perm = k;
-        perm_name(name, sizeof name, fmt, k);
-        if ((rc = write_header(name, &o, nid, nsnp))) goto done;
-        out = fopen(name, "a");
+        o.perm = perms ? perms[sc] : 0;
+        o.phenfile = phenfiles && phenfiles[sc] ? phenfiles[sc] : "";
+        if ((rc = write_header(outfiles[sc], &o, nid, nsnp))) goto done;
+        out = fopen(outfiles[sc], "a");
         if (!out) { rc = EPIH_ERR_IO; goto done; }
         int64_t t = 0;
         for (int i = 0, bi = 0; i < nchr; i++)
@@ -430,7 +455,7 @@ int epi_perform_scan_perms(const epi_run_opts *opt, const int32_t *geno, const c
         rep->new_hits += got;
         rep->total_hits += got;
         rep->blocks_done += nblk;
-        if (!opt->quiet) printf("permutation %d: %lld interactions -> %s\n", k, (long long)got, name);
+        if (!opt->quiet) printf("%lld interactions -> %s\n", (long long)got, outfiles[sc]);
     }
     if (!opt->quiet)
         printf("\nProgramme finished in %.1f s (device %.3f s)\n", now_s() - t0, rep->kernel_s);
@@ -440,9 +465,44 @@ done:
     if (ctx) epi_close(ctx);
     free(hits);
     free(blk);
-    free(phen);
     free(cs);
     free(cz);
     rep->wall_s = now_s() - t0;
+    return rc;
+}
+
+int epi_perform_scan_perms(const epi_run_opts *opt, const int32_t *geno, const chromosome *chrstat, const ped *dat,
+                           const map *genmap, int nid, int nsnp, int npack, int remain, int nchr, int first, int last,
+                           const char *fmt, epi_run_report *rep)
+{
+    epi_run_report local;
+    if (!rep) rep = &local;
+    memset(rep, 0, sizeof(*rep));
+    if (!opt || !dat || !fmt || nid < 2 || first < 1 || last < first) return EPIH_ERR_ARG;
+    const int np = last - first + 1;
+    float *phen = (float *)malloc(sizeof(float) * (size_t)nid * (size_t)np);
+    int *perms = (int *)malloc(sizeof(int) * (size_t)np);
+    char **names = (char **)calloc((size_t)np, sizeof(char *));
+    const char **pf = (const char **)calloc((size_t)np, sizeof(char *));
+    int rc = EPIH_OK;
+    if (!phen || !perms || !names || !pf) { rc = EPIH_ERR_IO; goto done; }
+    for (int k = first; k <= last; k++) {
+        const int s = k - first;
+        names[s] = (char *)malloc(4096);
+        if (!names[s]) { rc = EPIH_ERR_IO; goto done; }
+        if ((rc = perm_name(names[s], 4096, fmt, k))) goto done;
+        perms[s] = k;
+        pf[s] = opt->phenfile;
+        epi_permute_phen((unsigned)k, nid, dat, phen + (size_t)s * nid);
+    }
+    rc = epi_perform_scan_multi(opt, geno, chrstat, genmap, nid, nsnp, npack, remain, nchr, np, phen, perms, pf,
+                                (const char *const *)names, rep);
+done:
+    if (names)
+        for (int s = 0; s < np; s++) free(names[s]);
+    free(names);
+    free(pf);
+    free(perms);
+    free(phen);
     return rc;
 }

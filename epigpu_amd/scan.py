@@ -183,6 +183,11 @@ def _declare(name, L):
                                                 ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
                                                 _p(RunReport)]),
             "epi_permute_phen": (None, [ctypes.c_uint, ctypes.c_int, vp, vp]),
+            "epi_permute_order": (None, [ctypes.c_uint, ctypes.c_int, vp]),
+            "epi_write_perm_file": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, vp, vp, vp]),
+            "epi_perform_scan_multi": (ctypes.c_int, [_p(RunOpts), vp, vp, vp, ctypes.c_int, ctypes.c_int,
+                                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp,
+                                                      vp, vp, vp, _p(RunReport)]),
             "epi_perform_scan_perms": (ctypes.c_int, [_p(RunOpts), vp, vp, vp, vp, ctypes.c_int, ctypes.c_int,
                                                       ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                       ctypes.c_int, ctypes.c_char_p, _p(RunReport)]),
@@ -435,6 +440,48 @@ def permute_phen(seed, ped):
     out = np.zeros(len(ped), dtype=np.float32)
     epihost().epi_permute_phen(seed, len(ped), _ptr(ped), _ptr(out))
     return out
+
+
+def permute_order(seed, nid):
+    """Record order of permutation `seed`: order[i] = original index of the record at position i."""
+    out = np.zeros(nid, dtype=np.int32)
+    epihost().epi_permute_order(seed, nid, _ptr(out))
+    return out
+
+
+def write_perm_file(path, ped, order, phen=None):
+    """The reference's perm<k>.txt side file for a permutation order (phen by original index, or ped's)."""
+    ped = np.ascontiguousarray(ped)
+    order = np.ascontiguousarray(order, dtype=np.int32)
+    ph = None if phen is None else np.ascontiguousarray(phen, dtype=np.float32)
+    rc = epihost().epi_write_perm_file(os.fsencode(path), len(ped), _ptr(ped), _ptr(order), _ptr(ph))
+    if rc:
+        raise EpiError(rc, epihost().epih_strerror(rc).decode())
+
+
+def perform_scan_multi(words, nid, mp, chrs, phen, outfiles, perms=None, phenfiles=None, test="f", t1=6.5,
+                       t2=10.5, form="mfma", device=0, datafile="data.epigpu", quiet=True, safe=False):
+    """Several phenotype vectors (rows of phen) in one batched launch; scan s writes outfiles[s]."""
+    H = epihost()
+    o = RunOpts()
+    H.epi_run_opts_default(ctypes.byref(o))
+    o.test, o.t1, o.t2, o.device, o.form = ord(test), t1, t2, device, FORMS[form]
+    o.datafile, o.quiet, o.safe = datafile.encode(), int(quiet), int(safe)
+    rep = RunReport()
+    words = np.ascontiguousarray(words, dtype=np.int32)
+    phen = np.ascontiguousarray(np.atleast_2d(phen), dtype=np.float32)
+    n = phen.shape[0]
+    npack, remain, _ = layout(nid)
+    mp, chrs = (np.ascontiguousarray(x) for x in (mp, chrs))
+    names = (ctypes.c_char_p * n)(*[os.fsencode(f) for f in outfiles])
+    pf = (ctypes.c_char_p * n)(*[(x or "").encode() for x in (phenfiles or [""] * n)])
+    pm = np.ascontiguousarray(perms if perms is not None else np.zeros(n), dtype=np.int32)
+    rc = H.epi_perform_scan_multi(ctypes.byref(o), _ptr(words), _ptr(chrs), _ptr(mp), nid, words.shape[0], npack,
+                                  remain, len(chrs), n, _ptr(phen), _ptr(pm), ctypes.cast(pf, ctypes.c_void_p),
+                                  ctypes.cast(names, ctypes.c_void_p), ctypes.byref(rep))
+    if rc:
+        raise EpiError(rc, H.epih_strerror(rc).decode())
+    return {f: getattr(rep, f) for f, _ in RunReport._fields_}
 
 
 def perform_scan_perms(words, nid, ped, mp, chrs, fmt, first, last, test="f", t1=6.5, t2=10.5, form="mfma",

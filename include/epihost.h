@@ -238,19 +238,42 @@ int epi_perform_scan(const epi_run_opts *opt, const int32_t *geno,
                      int remain, int nchr, const char *filename,
                      epi_run_report *rep);
 
-/* Phenotypes of permutation k: the reference's permute() (src/main.c:336-357)
+/* Record order of permutation k: the reference's permute() (src/main.c:336-357)
  * shuffles the ped records with srand(k) and rand() % i from i = nid down to
- * 2 (src/main.c:309-313); phen_out receives the permuted records' phen. */
+ * 2 (src/main.c:309-313); order[i] receives the original index of the record
+ * that lands at position i. */
+void epi_permute_order(unsigned seed, int nid, int32_t *order);
+
+/* Phenotypes of permutation k: phen_out[i] = dat[order[i]].phen. */
 void epi_permute_phen(unsigned seed, int nid, const ped *dat, float *phen_out);
 
-/* Permutations first..last (1 <= first <= last) scanned together: the
- * reference runs one process per permutation (-p k, src/main.c:101-112,
- * :309-331); here every permutation is one scan of a single batched device
- * launch sharing the genotype bitplanes. Output file of permutation k is
- * fmt with its one %d replaced by k, written complete in the format of
- * epi_perform_scan with "Permutation: k" in its header. Existing output files
- * are refused (EPIH_ERR_DONE); this mode does not resume. opt->perm is
- * ignored. rep sums over all permutations. */
+/* The side file permute() writes, perm<k>.txt (src/main.c:349-356): a
+ * "Family ID Phenotype" line, then "family id phen" (phen with %f) of the
+ * permuted records in their new order. phen (may be NULL: dat[].phen) holds
+ * the phenotypes by original index, e.g. those of a -P file. */
+int epi_write_perm_file(const char *path, int nid, const ped *dat, const int32_t *order, const float *phen);
+
+/* Several phenotype vectors of one dataset scanned together in one batched
+ * device launch, sharing the genotype bitplanes: the reference runs one
+ * process per phenotype file (-P, src/main.c:172-181, :295-298) and per
+ * permutation (-p, :309-312). Scan s scans phen[s*nid .. s*nid + nid-1] and
+ * writes a complete output file outfiles[s] in the format of
+ * epi_perform_scan, whose header names phenfiles[s] (NULL or "": none) and
+ * permutation perms[s] (perms NULL: 0). Existing output files and repeated
+ * names are refused (EPIH_ERR_DONE, EPIH_ERR_ARG); this mode does not resume.
+ * opt->perm and opt->phenfile are ignored. rep sums over all scans. */
+int epi_perform_scan_multi(const epi_run_opts *opt, const int32_t *geno,
+                           const chromosome *chrstat, const map *genmap, int nid,
+                           int nsnp, int npack, int remain, int nchr, int nscan,
+                           const float *phen, const int *perms,
+                           const char *const *phenfiles, const char *const *outfiles,
+                           epi_run_report *rep);
+
+/* Permutations first..last (1 <= first <= last) of dat[].phen scanned
+ * together (epi_perform_scan_multi): the reference runs one process per
+ * permutation (-p k, src/main.c:101-112, :309-331). Output file of
+ * permutation k is fmt with its one %d replaced by k, with "Permutation: k"
+ * and opt->phenfile in its header. opt->perm is ignored. */
 int epi_perform_scan_perms(const epi_run_opts *opt, const int32_t *geno,
                            const chromosome *chrstat, const ped *dat,
                            const map *genmap, int nid, int nsnp, int npack,

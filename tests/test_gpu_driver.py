@@ -172,14 +172,19 @@ def test_cli_permutation_range(tmp_path, dataset):
     out = str(tmp_path / "cli.txt")
     exe = os.path.join(ROOT, "epigpu_amd", "lib", "episcan")
     r = subprocess.run([exe, "-A", data, out, "-t", "i", "-F", str(T1), "-I", str(T2), "-p", "2-3"],
-                       capture_output=True, text=True, timeout=120)
+                       capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
     assert r.returncode == 0, r.stderr
-    single = str(tmp_path / "single.txt")
+    one = tmp_path / "one"
+    one.mkdir()
+    single = str(one / "single.txt")
     r = subprocess.run([exe, "-A", data, single, "-t", "i", "-F", str(T1), "-I", str(T2), "-p", "3"],
-                       capture_output=True, text=True, timeout=120)
+                       capture_output=True, text=True, timeout=120, cwd=str(one))
     assert r.returncode == 0, r.stderr
     assert open(out + ".perm3").read() == open(single).read()
     assert os.path.exists(out + ".perm2") and not os.path.exists(out)
+    # the permute() side files (src/main.c:349-356), one per permutation, in the working directory
+    assert open(tmp_path / "perm3.txt").read() == open(one / "perm3.txt").read()
+    assert (tmp_path / "perm2.txt").exists()
 
 
 def test_safe_mode_cli_and_reference_main(tmp_path, dataset):

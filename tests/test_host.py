@@ -270,3 +270,78 @@ def test_perform_scan_perms_refuses_existing_outputs(tmp_path):
         S.perform_scan_perms(words, nid, ped, mp, chrs, str(tmp_path / "bad%s%d"), 1, 2)
     with pytest.raises(S.EpiError):
         S.perform_scan_perms(words, nid, ped, mp, chrs, fmt, 3, 1)
+
+
+def test_container_rejects_unterminated_names(tmp_path):
+    """Name fields are printed with %s, so a container whose name lacks a NUL within its 30 bytes, or whose
+    chromosome record points past the SNPs, is refused as malformed rather than read past its end."""
+    words, ped, mp, chrs = small_dataset()
+    cases = []
+    p1 = ped.copy()
+    p1[36]["id"] = b"X" * 30
+    cases.append((p1, mp, chrs))
+    m1 = mp.copy()
+    m1[8]["snpname"] = b"r" * 30
+    cases.append((ped, m1, chrs))
+    c1 = chrs.copy()
+    c1[1]["chrname"] = b"c" * 30
+    cases.append((ped, mp, c1))
+    c2 = chrs.copy()
+    c2[1]["chrsize"] = 100
+    cases.append((ped, mp, c2))
+    for k, (pp, mm, cc) in enumerate(cases):
+        path = str(tmp_path / f"bad{k}.epigpu")
+        E.write_container(path, words, 37, pp, mm, cc)
+        with pytest.raises(S.EpiError) as e:
+            E.read_container(path)
+        assert e.value.code == -102, k  # EPIH_ERR_FORMAT
+    good = str(tmp_path / "good.epigpu")
+    p2 = ped.copy()
+    p2[0]["id"] = b"Y" * 29  # 29 bytes and the terminator: the longest name
+    E.write_container(good, words, 37, p2, mp, chrs)
+    assert E.read_container(good)["ped"][0]["id"] == b"Y" * 29
+
+
+def test_permutation_side_file_is_the_references(tmp_path):
+    """perm<k>.txt as permute() writes it (src/main.c:349-356): a "Family ID Phenotype" line, then family, id
+    and %f phenotype of every record in its permuted position; the order is the shuffle restated with libc's
+    own srand/rand. Phenotypes of a -P file (by original index) replace the records' own."""
+    import ctypes.util
+    libc = ctypes.CDLL(ctypes.util.find_library("c"))
+    nid = 41
+    y = (np.arange(nid, dtype=np.float32) * 0.25 - 2.0).astype(np.float32)
+    ped, _, _ = S.make_records(nid, 4, [4], y)
+    for k in (1, 5):
+        order = S.permute_order(k, nid)
+        ref = list(range(nid))
+        libc.srand(k)
+        for i in range(nid, 1, -1):
+            r = libc.rand() % i
+            ref[r], ref[i - 1] = ref[i - 1], ref[r]
+        assert order.tolist() == ref
+        assert S.permute_phen(k, ped).tolist() == y[ref].tolist()
+        path = str(tmp_path / f"perm{k}.txt")
+        S.write_perm_file(path, ped, order)
+        lines = ["Family ID Phenotype\n"] + ["F%d I%d %f\n" % (j, j, y[j]) for j in ref]
+        assert open(path).read() == "".join(lines)
+        pfile = (np.arange(nid, dtype=np.float32) * -1.5).astype(np.float32)
+        S.write_perm_file(path, ped, order, pfile)
+        lines = ["Family ID Phenotype\n"] + ["F%d I%d %f\n" % (j, j, pfile[j]) for j in ref]
+        assert open(path).read() == "".join(lines)
+
+
+def test_perform_scan_multi_refuses_bad_outputs(tmp_path):
+    """The batched multi-phenotype driver refuses existing output files and repeated names before any device
+    work, so it runs without a GPU."""
+    nid, sizes = 64, [20]
+    words = S.synth_genotypes(3, nid, 20)
+    _, mp, chrs = S.make_records(nid, 20, sizes, S.synth_phenotype(3, 0, nid))
+    phen = np.stack([S.synth_phenotype(3, r, nid) for r in range(3)])
+    outs = [str(tmp_path / f"o{r}.txt") for r in range(3)]
+    open(outs[1], "w").write("x")
+    with pytest.raises(S.EpiError) as e:
+        S.perform_scan_multi(words, nid, mp, chrs, phen, outs)
+    assert e.value.code == -105  # EPIH_ERR_DONE
+    with pytest.raises(S.EpiError) as e:
+        S.perform_scan_multi(words, nid, mp, chrs, phen, [outs[0], outs[2], outs[0]])
+    assert e.value.code == -104  # EPIH_ERR_ARG
