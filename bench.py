@@ -101,8 +101,16 @@ def roofline(form, wl, kernel_ms, traffic=None, source=None):
                                   "FP32; 18*N logical flops per pair (9 cell-sum FMAs per individual); FAST tiles "
                                   "execute 4 of the 9 and derive the rest from per-SNP marginals")
     achieved = ops / (kernel_ms * 1e-3) / 1e12
-    return {"bound": bound, "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-            "traffic": traffic, "traffic_source": source, "ops_per_pair": ops / wl.pairs, "note": note}
+    out = {"bound": bound, "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+           "traffic": traffic, "traffic_source": source, "ops_per_pair": ops / wl.pairs, "note": note}
+    if form != "mfma":
+        # what the plain kernel executes: FAST tiles (no missing call) 4 cell FMAs per individual, GENERAL
+        # tiles 9 cell FMAs and one of y^2; a workload with missing calls has them in every tile
+        exe = 2.0 * wl.nid * (4 if wl.gmiss == 0 else 10)
+        out["executed_flops_per_pair"] = exe
+        out["executed_achieved"] = exe * wl.pairs / (kernel_ms * 1e-3) / 1e12
+        out["executed_frac"] = out["executed_achieved"] / peak
+    return out
 
 
 def accuracy(s, wl, form, phen=0):
