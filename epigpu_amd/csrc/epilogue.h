@@ -27,6 +27,7 @@ struct DevHit {
     double F, F2;
     int32_t tag, pad;  // tag: phenotype (single scan) or scan id (batch)
 };
+static_assert(sizeof(DevHit) == 40, "DevHit is the 40-byte epi_hit64 record");
 
 struct HitSink {
     DevHit *hits;

@@ -21,6 +21,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -122,6 +123,7 @@ struct epi_ctx {
     long long last_pairs = 0;
     int last_launches = 0;
     epi_schedule sched[2];  // per form
+    void *scratch = nullptr;  // this device's no-allocation scratch (address from hipGetSymbolAddress)
 };
 
 struct epi_batch {
@@ -582,6 +584,38 @@ PlainScan plain_scan(const Dataset &d, int r, int tag)
     return S;
 }
 
+// ------------------------------------------------------- no allocation --
+// Scratch of epi_scan_noalloc: the hit counter in the first 256 bytes, then
+// the tile list. Each device holds its own copy until the process ends. Host
+// code reaches it only through the address hipGetSymbolAddress returns on
+// that device (scratch_of), never through the array's name, which on the
+// host is a stand-in and no device address.
+constexpr size_t NOALLOC_BYTES = (size_t)16 << 20;
+constexpr size_t NOALLOC_TILES = (NOALLOC_BYTES - 256) / sizeof(int4);
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+std::mutex scratch_mu;
+std::vector<void *> scratch_ptr;  // per device ordinal
+
+// This device's scratch address, looked up once per device (the device must be current).
+int scratch_of(epi_ctx *c, void **out)
+{
+    std::lock_guard<std::mutex> g(scratch_mu);
+    if ((int)scratch_ptr.size() <= c->device) scratch_ptr.resize(c->device + 1, nullptr);
+    if (!scratch_ptr[c->device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(c, EPI_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        scratch_ptr[c->device] = ptr;
+    }
+    *out = scratch_ptr[c->device];
+    return EPI_OK;
+}
+
+// Clears the hit counter with a kernel (no memset node in a captured stream).
+__global__ void zero_counter(unsigned long long *n) { *n = 0ull; }
+
 // Schedules. FAST tiles (no missing call in either SNP range) are
 // fast_rows x 64. GENERAL tiles are fast_rows x 32 in the MFMA form (the
 // same workgroup size as its FAST tiles, two waves per SIMD) and 16 x 64 in
@@ -768,28 +802,38 @@ int check_params(epi_ctx *c, const epi_scan_params *p)
         }                                                                                                \
     } while (0)
 
+// Device buffers of a scan's pair kernels: the tile list, the hit records and the hit counter.
+struct ScanBufs {
+    int4 *tiles;
+    DevHit *hits;
+    unsigned long long *nhits;
+    long long cap;
+};
+
+ScanBufs ctx_bufs(const epi_ctx *c) { return {c->tiles, c->hits, c->nhits, c->cap}; }
+
 // Launch the pair kernels of one scan over the uploaded tile lists (fast tiles first).
 template <bool DUMP>
 int launch_pairs(epi_ctx *c, const epi_scan_params *p, const epi_schedule &sc, const TileLists &tl,
-                 const DumpSink &ds)
+                 const DumpSink &ds, const ScanBufs &B)
 {
     const Dataset &d = c->d;
     if (tl.fast.size() + tl.gen.size() > (size_t)INT32_MAX)
         return fail(c, EPI_ERR_RANGE, "a super-tile row of this scan exceeds 2^31 tiles");
     const int nf = (int)tl.fast.size(), ng = (int)tl.gen.size();
     c->last_launches = 0;
-    const HitSink hs = make_sink(c->hits, c->nhits, c->cap, p);
+    const HitSink hs = make_sink(B.hits, B.nhits, B.cap, p);
     if (p->form == EPI_FORM_MFMA) {
         MfmaArgs A;
         A.S = mfma_scan(d, p->phen, p->phen);
         A.hs = hs;
         A.ds = ds;
         if (sc.fast_rows == 64) {
-            LAUNCH_TILES((mfma_fast_kernel<64, DUMP>), fast_threads<64>(), A, c->tiles, nf);
-            LAUNCH_TILES((mfma_general_kernel<64, DUMP>), general_threads<64>(), A, c->tiles + nf, ng);
+            LAUNCH_TILES((mfma_fast_kernel<64, DUMP>), fast_threads<64>(), A, B.tiles, nf);
+            LAUNCH_TILES((mfma_general_kernel<64, DUMP>), general_threads<64>(), A, B.tiles + nf, ng);
         } else {
-            LAUNCH_TILES((mfma_fast_kernel<32, DUMP>), fast_threads<32>(), A, c->tiles, nf);
-            LAUNCH_TILES((mfma_general_kernel<32, DUMP>), general_threads<32>(), A, c->tiles + nf, ng);
+            LAUNCH_TILES((mfma_fast_kernel<32, DUMP>), fast_threads<32>(), A, B.tiles, nf);
+            LAUNCH_TILES((mfma_general_kernel<32, DUMP>), general_threads<32>(), A, B.tiles + nf, ng);
         }
     } else {
         PlainArgs P;
@@ -797,10 +841,10 @@ int launch_pairs(epi_ctx *c, const epi_scan_params *p, const epi_schedule &sc, c
         P.hs = hs;
         P.ds = ds;
         if (sc.fast_rows == 32)
-            LAUNCH_TILES((plain_kernel<2, 4, true, DUMP>), 256, P, c->tiles, nf);
+            LAUNCH_TILES((plain_kernel<2, 4, true, DUMP>), 256, P, B.tiles, nf);
         else
-            LAUNCH_TILES((plain_kernel<1, 4, true, DUMP>), 256, P, c->tiles, nf);
-        LAUNCH_TILES((plain_kernel<1, 4, false, DUMP>), 256, P, c->tiles + nf, ng);
+            LAUNCH_TILES((plain_kernel<1, 4, true, DUMP>), 256, P, B.tiles, nf);
+        LAUNCH_TILES((plain_kernel<1, 4, false, DUMP>), 256, P, B.tiles + nf, ng);
     }
     return EPI_OK;
 }
@@ -1104,7 +1148,7 @@ int run_scan(epi_ctx *c, const epi_scan_params *p, const epi_schedule &sc)
     long long pairs = tl.pairs;
     int launches = 0;
     for (;;) {
-        if ((rc = launch_pairs<false>(c, p, sc, tl, ds))) return rc;
+        if ((rc = launch_pairs<false>(c, p, sc, tl, ds, ctx_bufs(c)))) return rc;
         launches += c->last_launches;
         if (done) break;
         // the launched kernels read the device tile list that the next piece overwrites
@@ -1242,6 +1286,10 @@ int epi_open(int device, epi_ctx **out)
         epi_close(c);
         return EPI_ERR_NOMEM;
     }
+    if (scratch_of(c, &c->scratch)) {
+        epi_close(c);
+        return EPI_ERR_HIP;
+    }
     *out = c;
     return EPI_OK;
 }
@@ -1268,13 +1316,25 @@ int epi_device_name(const epi_ctx *c, char *buf, size_t n)
     return EPI_OK;
 }
 
+// Marginals as soon as genotypes and phenotypes are both set, so that scans
+// (epi_scan_noalloc among them) find them ready. A failure drops the
+// phenotypes.
+int ready_marginals(epi_ctx *c)
+{
+    if (!c->d.mbits || c->d.ph.empty()) return EPI_OK;
+    const int rc = ds_marginals(c, c->d);
+    if (rc) drop_phenotypes(c->d);
+    return rc;
+}
+
 int epi_set_genotypes(epi_ctx *c, const int32_t *words, int32_t nid, int32_t nsnp)
 {
     if (!c) return EPI_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
     c->last_nhits = 0;
     c->tl_valid = false;
-    return ds_set_genotypes(c, c->d, words, nid, nsnp);
+    const int rc = ds_set_genotypes(c, c->d, words, nid, nsnp);
+    return rc ? rc : ready_marginals(c);
 }
 
 int epi_set_phenotypes_ex(epi_ctx *c, const float *phen, int32_t nphen, int32_t nid, int32_t flags)
@@ -1282,7 +1342,8 @@ int epi_set_phenotypes_ex(epi_ctx *c, const float *phen, int32_t nphen, int32_t 
     if (!c) return EPI_ERR_ARG;
     if (flags & ~EPI_PHEN_NINE_IS_VALUE) return fail(c, EPI_ERR_ARG, "unknown phenotype flags");
     HIPCHK(c, hipSetDevice(c->device));
-    return ds_set_phenotypes(c, c->d, phen, nphen, nid, flags);
+    const int rc = ds_set_phenotypes(c, c->d, phen, nphen, nid, flags);
+    return rc ? rc : ready_marginals(c);
 }
 
 int epi_set_phenotypes(epi_ctx *c, const float *phen, int32_t nphen, int32_t nid)
@@ -1337,6 +1398,54 @@ int epi_scan(epi_ctx *c, const epi_scan_params *p)
     HIPCHK(c, hipSetDevice(c->device));
     if ((rc = ds_marginals(c, c->d))) return rc;
     return run_scan(c, p, c->sched[p->form]);
+}
+
+int epi_scan_noalloc(epi_ctx *c, const epi_scan_params *p, epi_hit64 *hits, int64_t cap, int64_t *nhits)
+{
+    if (!c) return EPI_ERR_ARG;
+    int rc = check_params(c, p);
+    if (rc) return rc;
+    if (!hits || cap < 0 || !nhits) return fail(c, EPI_ERR_ARG, "need a device hit buffer, its capacity and nhits");
+    if (!c->d.marg_ok) return fail(c, EPI_ERR_STATE, "genotypes and phenotypes must both be set");
+    if (!c->scratch) return fail(c, EPI_ERR_STATE, "no scratch address for this device");
+    HIPCHK(c, hipSetDevice(c->device));
+    const epi_schedule &sc = c->sched[p->form];
+    const int nchr = (int)c->d.chrstart.size(), nblk = nchr * (nchr + 1) / 2;
+    const int bb = std::max(0, p->blk_begin), be = p->blk_end <= 0 ? nblk : std::min(nblk, p->blk_end);
+    TileLists tl;
+    TileCursor cur;
+    cur.blk = bb;
+    const bool done = build_tiles(c->d, be, p->form, sc, NOALLOC_TILES + 1, cur, tl);
+    const size_t nt = tl.fast.size() + tl.gen.size();
+    if (!done || nt > NOALLOC_TILES)
+        return fail(c, EPI_ERR_RANGE,
+                    "the scan needs more than the %zu tiles the no-allocation scratch holds (%zu MiB); use epi_scan",
+                    NOALLOC_TILES, NOALLOC_BYTES >> 20);
+    unsigned char *base = static_cast<unsigned char *>(c->scratch);
+    const ScanBufs B = {reinterpret_cast<int4 *>(base + 256), reinterpret_cast<DevHit *>(hits),
+                        reinterpret_cast<unsigned long long *>(base), (long long)cap};
+    if (!tl.fast.empty())
+        HIPCHK(c, hipMemcpyAsync(B.tiles, tl.fast.data(), tl.fast.size() * sizeof(int4), hipMemcpyHostToDevice,
+                                 c->stream));
+    if (!tl.gen.empty())
+        HIPCHK(c, hipMemcpyAsync(B.tiles + tl.fast.size(), tl.gen.data(), tl.gen.size() * sizeof(int4),
+                                 hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(zero_counter, dim3(1), dim3(1), 0, c->stream, B.nhits);
+    HIPCHK(c, hipGetLastError());
+    DumpSink ds;
+    memset(&ds, 0, sizeof(ds));
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    if ((rc = launch_pairs<false>(c, p, sc, tl, ds, B))) return rc;
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    unsigned long long nh = 0;
+    HIPCHK(c, hipMemcpyAsync(&nh, B.nhits, sizeof(nh), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    c->last_pairs = tl.pairs;
+    *nhits = (int64_t)nh;
+    return EPI_OK;
 }
 
 int epi_autotune(epi_ctx *c, const epi_scan_params *p, int32_t dims, int32_t reps, epi_schedule *best,
@@ -1474,7 +1583,7 @@ int epi_pair_dump(epi_ctx *c, const epi_scan_params *p, int32_t a0, int32_t a1, 
     h2d(dsum, sums, (size_t)np * 9 * 8);
     h2d(dF, F, (size_t)np * 8);
     h2d(dF2, F2, (size_t)np * 8);
-    if (hrc == EPI_OK) hrc = launch_pairs<true>(c, p, sc, all, ds);
+    if (hrc == EPI_OK) hrc = launch_pairs<true>(c, p, sc, all, ds, ctx_bufs(c));
     auto d2h = [&](void *hdst, const void *d, size_t bytes) {
         if (hrc == EPI_OK && hdst && hipMemcpyAsync(hdst, d, bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
             hrc = fail(c, EPI_ERR_HIP, "dump copy back failed");

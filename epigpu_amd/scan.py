@@ -96,6 +96,8 @@ class Container(ctypes.Structure):
 
 
 # Record layouts of the epiGPU container (src/readplink.h:17-39, x86-64 gcc).
+HIT64_DTYPE = np.dtype([("snp1", "<i4"), ("snp2", "<i4"), ("df1", "<i4"), ("df2", "<i4"), ("F", "<f8"),
+                        ("F2", "<f8"), ("tag", "<i4"), ("pad", "<i4")])
 PED_DTYPE = np.dtype({"names": ["family", "id", "paternal", "maternal", "sex", "phen"],
                       "formats": ["S30", "S30", "S30", "S30", "S1", "<f4"],
                       "offsets": [0, 30, 60, 90, 120, 124], "itemsize": 128})
@@ -138,6 +140,7 @@ def _declare(name, L):
             "epi_set_chromosomes": (ctypes.c_int, [vp, vp, vp, i32]),
             "epi_scan_params_default": (None, [_p(ScanParams)]),
             "epi_scan": (ctypes.c_int, [vp, _p(ScanParams)]),
+            "epi_scan_noalloc": (ctypes.c_int, [vp, _p(ScanParams), vp, i64, _p(i64)]),
             "epi_hit_count": (ctypes.c_int, [vp, _p(i64), _p(i32)]),
             "epi_get_hits": (ctypes.c_int, [vp, vp, vp, vp, vp, i64, _p(i64)]),
             "epi_set_hit_capacity": (ctypes.c_int, [vp, i64]),
@@ -638,6 +641,16 @@ class Scanner:
             g = got.value
             out.update(hits=hits[:g], F64=F[:g], F2_64=F2[:g], block=blk[:g])
         return out
+
+    def scan_noalloc(self, hits_dev, cap, test="f", form="mfma", t1=6.5, t2=10.5, phen=0, blk_begin=0, blk_end=0,
+                     safe=False):
+        """epi_scan_noalloc: hits (HIT64_DTYPE records, unsorted) into the device buffer at address hits_dev of
+        cap records; returns the total hit count (may exceed cap)."""
+        p = self.params(test, form, t1, t2, phen, blk_begin, blk_end, safe)
+        n = ctypes.c_int64()
+        self._chk(self.L.epi_scan_noalloc(self.ctx, ctypes.byref(p), ctypes.c_void_p(hits_dev), cap,
+                                          ctypes.byref(n)), "epi_scan_noalloc")
+        return n.value
 
     def pair_dump(self, a0, a1, b0, b1, test="i", form="mfma", phen=0):
         p = self.params(test, form, 6.5, 10.5, phen)

@@ -117,6 +117,36 @@ int epi_hit_count(epi_ctx *ctx, int64_t *nhits, int32_t *overflow);
 int epi_get_hits(epi_ctx *ctx, epi_hit *hits, double *F64, double *F2_64, int32_t *block,
                  int64_t cap, int64_t *n);
 
+/* FP64 hit record of a device hit buffer (40 bytes): the fields of the
+ * reference's `table` (src/id4117.h:14-21) with F and F2 in FP64, then the
+ * phenotype index and padding. */
+typedef struct epi_hit64 {
+    int32_t snp1;
+    int32_t snp2;
+    int32_t df1;
+    int32_t df2;
+    double F;
+    double F2;
+    int32_t tag;
+    int32_t pad;
+} epi_hit64;
+
+/* epi_scan without any device allocation. It has no counterpart in the
+ * reference; it is for callers that cannot allocate device memory, such as
+ * tight serving loops and code that must not call hipMalloc. The tile list
+ * and the hit counter live in a fixed 16 MiB device array of the library
+ * (each device holds its own until the process ends), so a scan may need at
+ * most 1,048,560 tiles: about 64,000 SNPs in one block without missing calls,
+ * or 45,000 with them, at 32-row tiles. A larger scan fails with
+ * EPI_ERR_RANGE before any device work. Genotypes and phenotypes must be set
+ * (their uploads allocate). The hits, unsorted, go to the caller's device
+ * buffer `hits` of `cap` records; *nhits receives their total, which may
+ * exceed cap (records beyond it are not written). Synchronous; the hit
+ * buffer of epi_get_hits is not touched. Calls that overlap on one device
+ * share the array and race. */
+int epi_scan_noalloc(epi_ctx *ctx, const epi_scan_params *p, epi_hit64 *hits, int64_t cap,
+                     int64_t *nhits);
+
 /* Device capacity of the hit buffer (records); default 1<<22. */
 int epi_set_hit_capacity(epi_ctx *ctx, int64_t cap);
 
