@@ -149,7 +149,9 @@ __device__ __forceinline__ bool screen32(const int n[9], const float s[9], float
         ssb += (float)n[c] * e * e;
     }
     float ssw = sq - (float)N * mbar * mbar - ssb;
-    if (!(ssw > 0.0f)) return true;  // degenerate: let FP64 decide
+    // degenerate, or SSW a small difference of large terms (an outlier alone in its cell): FP32 cannot
+    // decide within the 2% margin, so FP64 does
+    if (!(ssw > 1e-4f * (sq + (float)N * mbar * mbar + ssb))) return true;
     // F > t  <=>  ssb * d2 > t * d1 * ssw
     if (ssb * (float)d2 > t1s * (float)d1 * ssw) return true;
     if (test == 'f') return false;

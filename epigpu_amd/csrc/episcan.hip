@@ -71,6 +71,7 @@ struct PhenDev {
     double sqV = 0.0;  // sum q^2
     double ysV = 0.0;  // sum yc
     double y2V = 0.0;  // sum yc^2
+    double ouSum = 0.0, ouSq = 0.0;  // sum of the outliers' values (units 2^-fbits) and of their squares
     bool mfma_ok = true;
     std::string mfma_why;
 };
@@ -428,6 +429,8 @@ int encode_phen(epi_ctx *c, const float *y, int nid, int Kp, bool nine_value, in
         const double u = std::ldexp((double)y[e.oidx[t]] - centre, f);
         e.ov.d[t] = u;
         e.ov.f[t] = (float)u;
+        p.ouSum += u;
+        p.ouSq += u * u;
     }
     p.fbits = f;
     p.zbits = zb;
@@ -564,6 +567,8 @@ MfmaScan mfma_scan(const Dataset &d, int r, int tag)
     S.ocodes = p.ocodes;
     S.ov = p.ov;
     S.nout = p.nout;
+    S.ouSum = p.ouSum;
+    S.ouSq = p.ouSq;
     return S;
 }
 

@@ -78,7 +78,32 @@ struct MfmaScan {                // one dataset x one phenotype
     double qVd;                  // the same, as a double (exact)
     double sqV;                  // sum q^2 over present (outliers excluded)
     double zunit;                // 2^zbits: q^2 units per unit of the Z limbs
+    double ouSum, ouSq;          // sum of the outliers' values and of their squares
 };
+
+// Class of outlier 16w+t at a SNP from its code word w (3: no call).
+__device__ __forceinline__ uint32_t out_class(uint32_t code, int t) { return (code >> (2 * t)) & 3u; }
+
+// FAST tiles (no missing call in the tile): every outlier is a complete case of
+// every pair, so only the four cells of classes {0,1} x {0,1} depend on the
+// pair. Adds sum_j u_j [g_a(j) = g][g_b(j) = h] to x[g][h] (FP64 path).
+__device__ __forceinline__ void add_outlier_cells(const MfmaScan &S, int a, int b, double x[3][3])
+{
+    const int an = a < S.nsnp ? a : S.nsnp - 1, bn = b < S.nsnp ? b : S.nsnp - 1;
+    for (int w = 0; 16 * w < S.nout; w++) {
+        const uint32_t ca = S.ocodes[(size_t)w * S.nsnp + an], cb = S.ocodes[(size_t)w * S.nsnp + bn];
+        const int nt = S.nout - 16 * w < 16 ? S.nout - 16 * w : 16;
+        for (int t = 0; t < nt; t++) {
+            const uint32_t ga = out_class(ca, t), gb = out_class(cb, t);
+            const double u = S.ov->d[16 * w + t];
+            const double u0 = ga == 0u ? u : 0.0, u1 = ga == 1u ? u : 0.0;
+            x[0][0] += gb == 0u ? u0 : 0.0;
+            x[0][1] += gb == 1u ? u0 : 0.0;
+            x[1][0] += gb == 0u ? u1 : 0.0;
+            x[1][1] += gb == 1u ? u1 : 0.0;
+        }
+    }
+}
 
 // Adds the outlier individuals that are complete cases of pair (a, b) to its
 // {0,1,NM}^2 table x of sums and to the sum of squares sq (fixed-point units).
@@ -92,7 +117,7 @@ __device__ __forceinline__ void add_outliers(const MfmaScan &S, int a, int b, T 
         const uint32_t ca = S.ocodes[(size_t)w * S.nsnp + an], cb = S.ocodes[(size_t)w * S.nsnp + bn];
         const int nt = S.nout - 16 * w < 16 ? S.nout - 16 * w : 16;
         for (int t = 0; t < nt; t++) {
-            const uint32_t ga = (ca >> (2 * t)) & 3u, gb = (cb >> (2 * t)) & 3u;
+            const uint32_t ga = out_class(ca, t), gb = out_class(cb, t);
             T v;
             if constexpr (sizeof(T) == 8)
                 v = S.ov->d[16 * w + t];
@@ -453,15 +478,80 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
         int sn = tid < TA ? a0 + tid : b0 + tid - TA;
         sn = sn < A.nsnp ? sn : A.nsnp - 1;
         const Marg &mm = A.marg[sn];
+        // the outliers of each class at this SNP (all of them complete cases in a FAST tile)
+        double oq[2] = {0.0, 0.0};
+        for (int w = 0; 16 * w < A.nout; w++) {
+            const uint32_t c = A.ocodes[(size_t)w * A.nsnp + sn];
+            const int nt = A.nout - 16 * w < 16 ? A.nout - 16 * w : 16;
+            for (int t = 0; t < nt; t++) {
+                const uint32_t g = out_class(c, t);
+                const double u = A.ov->d[16 * w + t];
+                oq[0] += g == 0u ? u : 0.0;
+                oq[1] += g == 1u ? u : 0.0;
+            }
+        }
 #pragma unroll
         for (int g = 0; g < 2; g++) {
             mg_n[2 * tid + g] = mm.n[g];
-            mg_q[2 * tid + g] = (double)mm.q[g];
-            mg_f[2 * tid + g] = (float)mm.q[g];
+            mg_q[2 * tid + g] = (double)mm.q[g] + oq[g];
+            mg_f[2 * tid + g] = (float)((double)mm.q[g] + oq[g]);
         }
     }
     __syncthreads();
-    const float sqV = (float)A.sqV, qVf = (float)A.qV;
+    const float sqV = (float)(A.sqV + A.ouSq), qVf = (float)(A.qVd + A.ouSum);
+
+    // Outliers in the cells {0,1} x {0,1} of the lane's eight pairs, in one
+    // wave-uniform loop: corr[i][p][bg][2g+h].
+    float corr[MB][2][2][4];
+#pragma unroll
+    for (int i = 0; i < MB; i++)
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int bg = 0; bg < 2; bg++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) corr[i][p][bg][c] = 0.0f;
+    if (A.nout) {
+        for (int w = 0; 16 * w < A.nout; w++) {
+            uint32_t ca[MB][2], cb[2];
+#pragma unroll
+            for (int i = 0; i < MB; i++)
+#pragma unroll
+                for (int p = 0; p < 2; p++) {
+                    const int a = a0 + APW * wid + 4 * i + 2 * hf + p;
+                    ca[i][p] = A.ocodes[(size_t)w * A.nsnp + (a < A.nsnp ? a : A.nsnp - 1)];
+                }
+#pragma unroll
+            for (int bg = 0; bg < 2; bg++) {
+                const int b = b0 + 32 * bg + m;
+                cb[bg] = A.ocodes[(size_t)w * A.nsnp + (b < A.nsnp ? b : A.nsnp - 1)];
+            }
+            const int nt = A.nout - 16 * w < 16 ? A.nout - 16 * w : 16;
+            for (int t = 0; t < nt; t++) {
+                const float u = A.ov->f[16 * w + t];
+                float ub[2][2];
+#pragma unroll
+                for (int bg = 0; bg < 2; bg++) {
+                    const uint32_t gb = out_class(cb[bg], t);
+                    ub[bg][0] = gb == 0u ? u : 0.0f;
+                    ub[bg][1] = gb == 1u ? u : 0.0f;
+                }
+#pragma unroll
+                for (int i = 0; i < MB; i++)
+#pragma unroll
+                    for (int p = 0; p < 2; p++) {
+                        const uint32_t ga = out_class(ca[i][p], t);
+#pragma unroll
+                        for (int bg = 0; bg < 2; bg++)
+#pragma unroll
+                            for (int h = 0; h < 2; h++) {
+                                corr[i][p][bg][h] += ga == 0u ? ub[bg][h] : 0.0f;
+                                corr[i][p][bg][2 + h] += ga == 1u ? ub[bg][h] : 0.0f;
+                            }
+                    }
+            }
+        }
+    }
 
     // Epilogue: lane holds pairs (a = a0 + APW w + 4i + 2hf + p, b = b0 + 32bg + m);
     // acc[i][2bg+h][8p + 4g + w] = X[g][h][w].
@@ -484,7 +574,8 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
                         const v16i &v = acc[i][2 * bg + h];
                         const int e = 8 * p + 4 * g;
                         xc[g][h] = v[e];
-                        xf[g][h] = (float)v[e + 1] + 256.0f * (float)v[e + 2] + 65536.0f * (float)v[e + 3];
+                        xf[g][h] = (float)v[e + 1] + 256.0f * (float)v[e + 2] + 65536.0f * (float)v[e + 3] +
+                                   corr[i][p][bg][2 * g + h];
                     }
                     xc[g][2] = mg_n[2 * al + g];
                     xf[g][2] = mg_f[2 * al + g];
@@ -493,8 +584,7 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
                 }
                 xc[2][2] = A.nV;
                 xf[2][2] = qVf;
-                float sqf = sqV;
-                if (A.nout) add_outliers(A, a, b, xf, sqf);
+                const float sqf = sqV;
                 int n[9];
                 float sf[9];
                 cells_from_x(xc, n);
@@ -513,9 +603,9 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
                         xs[g][2] = mg_q[2 * al + g];
                         xs[2][g] = mg_q[2 * bl + g];
                     }
-                    xs[2][2] = A.qVd;
-                    sq = A.sqV;
-                    if (A.nout) add_outliers(A, a, b, xs, sq);
+                    xs[2][2] = A.qVd + A.ouSum;
+                    sq = A.sqV + A.ouSq;
+                    if (A.nout) add_outlier_cells(A, a, b, xs);
                     cells_from_x(xs, sd);
                 };
                 const bool valid = a < a1 && b < b1 && a > b;
