@@ -42,7 +42,7 @@ constexpr int MAX_NID = 65535 / (KPAD / 32) * KPAD;
 constexpr long long QMAX = 8355711;     // largest |q| with three balanced int8 limbs
 constexpr long long ZMAX = 2139062143;  // largest |z| with four balanced int8 limbs
 // The MFMA form's phenotype range. The central SD s_c is the SD of the
-// present values without the 2 OUT_MAX farthest from the mean (a quarter of
+// present values without the 2 OUT_MAX farthest from the median (a quarter of
 // them when there are fewer than 258). Among the OUT_MAX farthest, values more
 // than OUT_SD s_c from the central mean are outliers, carried exactly. The
 // encoded values must then lie within BULK_SD s_c of the centre, which bounds
@@ -72,6 +72,7 @@ struct PhenDev {
     double ysV = 0.0;  // sum yc
     double y2V = 0.0;  // sum yc^2
     double ouSum = 0.0, ouSq = 0.0;  // sum of the outliers' values (units 2^-fbits) and of their squares
+    double sc = 0.0, maxdev = 0.0;   // central SD; largest |y - centre| of the encoded values
     bool mfma_ok = true;
     std::string mfma_why;
 };
@@ -333,11 +334,12 @@ int encode_phen(epi_ctx *c, const float *y, int nid, int Kp, bool nine_value, in
     const int nv = (int)P.size();
     if (nv < 2) return fail(c, EPI_ERR_RANGE, "phenotype %d has fewer than 2 present values", r);
     PhenDev &p = e.p;
-    // Outliers among the ntop values farthest from the mean, measured against
-    // the central values (all but the ncen farthest).
-    double c0 = 0.0;
-    for (int k : P) c0 += (double)y[k];
-    c0 /= nv;
+    // Outliers among the ntop values farthest from the median (which far values
+    // cannot drag), measured against the central values (all but the ncen farthest).
+    std::vector<float> med(nv);
+    for (int t = 0; t < nv; t++) med[t] = y[P[t]];
+    std::nth_element(med.begin(), med.begin() + nv / 2, med.end());
+    const double c0 = (double)med[nv / 2];
     const int ncen = std::min(2 * OUT_MAX, (nv - 2) / 4), ntop = std::min(OUT_MAX, ncen);
     std::vector<int> ord(P);
     auto farther = [&](int x, int z) {
@@ -375,6 +377,8 @@ int encode_phen(epi_ctx *c, const float *y, int nid, int Kp, bool nine_value, in
         maxall = std::max(maxall, dv);
         if (!out[k]) maxabs = std::max(maxabs, dv);
     }
+    p.sc = sb;
+    p.maxdev = maxabs;
     p.mfma_ok = maxabs <= BULK_SD * sb;
     if (!p.mfma_ok) {
         char buf[256];
@@ -1354,6 +1358,26 @@ int epi_set_phenotypes_ex(epi_ctx *c, const float *phen, int32_t nphen, int32_t 
 int epi_set_phenotypes(epi_ctx *c, const float *phen, int32_t nphen, int32_t nid)
 {
     return epi_set_phenotypes_ex(c, phen, nphen, nid, 0);
+}
+
+int epi_phen_encoding(const float *phen, int32_t nid, int32_t flags, epi_phen_info *info)
+{
+    if (!phen || !info || nid < 2 || nid > MAX_NID || (flags & ~EPI_PHEN_NINE_IS_VALUE)) return EPI_ERR_ARG;
+    PhenEnc e;
+    const int Kp = (nid + KPAD - 1) / KPAD * KPAD;
+    const int rc = encode_phen(nullptr, phen, nid, Kp, (flags & EPI_PHEN_NINE_IS_VALUE) != 0, 0, e);
+    if (rc) return rc;
+    memset(info, 0, sizeof(*info));
+    info->present = e.p.nV;
+    info->outliers = e.p.nout;
+    info->mfma_ok = e.p.mfma_ok ? 1 : 0;
+    info->fbits = e.p.fbits;
+    info->zbits = e.p.zbits;
+    info->centre = e.p.centre;
+    info->central_sd = e.p.sc;
+    info->max_dev = e.p.maxdev;
+    for (int t = 0; t < e.p.nout; t++) info->outlier_index[t] = e.oidx[t];
+    return EPI_OK;
 }
 
 int epi_set_chromosomes(epi_ctx *c, const int32_t *chrstart, const int32_t *chrsize, int32_t nchr)

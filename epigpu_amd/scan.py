@@ -36,6 +36,14 @@ HIT_DTYPE = np.dtype([("snp1", "<i4"), ("snp2", "<i4"), ("df1", "<i4"), ("df2", 
                       ("F", "<f4"), ("F2", "<f4")])
 
 
+class PhenInfo(ctypes.Structure):
+    """epi_phen_info: how the MFMA form encodes a phenotype vector."""
+    _fields_ = [("present", ctypes.c_int32), ("outliers", ctypes.c_int32), ("mfma_ok", ctypes.c_int32),
+                ("fbits", ctypes.c_int32), ("zbits", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("centre", ctypes.c_double), ("central_sd", ctypes.c_double), ("max_dev", ctypes.c_double),
+                ("outlier_index", ctypes.c_int32 * 32)]
+
+
 class ScanParams(ctypes.Structure):
     _fields_ = [("test", ctypes.c_int32), ("form", ctypes.c_int32), ("t1", ctypes.c_float),
                 ("t2", ctypes.c_float), ("phen", ctypes.c_int32), ("blk_begin", ctypes.c_int32),
@@ -140,6 +148,7 @@ def _declare(name, L):
             "epi_set_chromosomes": (ctypes.c_int, [vp, vp, vp, i32]),
             "epi_scan_params_default": (None, [_p(ScanParams)]),
             "epi_scan": (ctypes.c_int, [vp, _p(ScanParams)]),
+            "epi_phen_encoding": (ctypes.c_int, [vp, i32, i32, _p(PhenInfo)]),
             "epi_scan_noalloc": (ctypes.c_int, [vp, _p(ScanParams), vp, i64, _p(i64)]),
             "epi_hit_count": (ctypes.c_int, [vp, _p(i64), _p(i32)]),
             "epi_get_hits": (ctypes.c_int, [vp, vp, vp, vp, vp, i64, _p(i64)]),
@@ -279,6 +288,19 @@ def f_isf(p, d1, d2):
 
 def betai(a, b, x):
     return epihost().epi_betai(a, b, x)
+
+
+def phen_encoding(phen, nine_is_value=False):
+    """How the MFMA form encodes one phenotype vector (epi_phen_encoding; host only)."""
+    y = np.ascontiguousarray(phen, dtype=np.float32)
+    info = PhenInfo()
+    rc = episcan().epi_phen_encoding(_ptr(y), len(y), PHEN_NINE_IS_VALUE if nine_is_value else 0,
+                                     ctypes.byref(info))
+    if rc:
+        raise EpiError(rc, "epi_phen_encoding: " + episcan().epi_strerror(rc).decode())
+    out = {f: getattr(info, f) for f, _ in PhenInfo._fields_ if f not in ("pad", "outlier_index")}
+    out["outlier_index"] = list(info.outlier_index)[:info.outliers]
+    return out
 
 
 def read_container(path):

@@ -79,6 +79,32 @@ int epi_set_phenotypes(epi_ctx *ctx, const float *phen, int32_t nphen, int32_t n
 int epi_set_phenotypes_ex(epi_ctx *ctx, const float *phen, int32_t nphen, int32_t nid,
                           int32_t flags);
 
+/* How the MFMA form encodes one phenotype vector (host only, no device
+ * needed). Replaces nothing in the reference, whose kernels take FP32 values
+ * (src/EpiOpenCL.c:135-147): the MFMA form holds phenotypes as integers
+ * q = round((y - centre) * 2^fbits) in three signed 8-bit limbs. The central
+ * SD is that of the present values without the 64 farthest from their median
+ * (a quarter of them below 258 present values). Among the 32 farthest,
+ * values more than 32 central SDs out are outliers, carried exactly beside
+ * the integers; the rest must lie within 64 central SDs of their centre,
+ * or MFMA scans of this phenotype return EPI_ERR_RANGE (mfma_ok = 0) and the
+ * plain form takes it. Returns EPI_ERR_RANGE for an infinite value or fewer
+ * than two present values, as epi_set_phenotypes does. */
+typedef struct epi_phen_info {
+    int32_t present;            /* values present (not -9 or NaN) */
+    int32_t outliers;           /* values carried exactly, 0..32 */
+    int32_t mfma_ok;            /* 1: the MFMA form holds F, F2 to 1e-5 */
+    int32_t fbits;              /* q = round((y - centre) * 2^fbits) */
+    int32_t zbits;              /* the sum of squares uses round(q^2 / 2^zbits) */
+    int32_t pad;
+    double centre;              /* mean of the encoded (non-outlier) values */
+    double central_sd;
+    double max_dev;             /* largest |y - centre| among the encoded values */
+    int32_t outlier_index[32];  /* individuals of the outliers */
+} epi_phen_info;
+
+int epi_phen_encoding(const float *phen, int32_t nid, int32_t flags, epi_phen_info *info);
+
 /* Chromosome blocks (chromosome.chrstart / chrsize, src/readplink.h:34-39).
  * Blocks are numbered in the reference's order: block(i, j) = i(i+1)/2 + j
  * for j <= i (src/EpiOpenCL.c:1527-1540, :1445-1456). */
