@@ -135,8 +135,9 @@ struct epi_batch {
     MfmaScan *d_mscan = nullptr;
     PlainScan *d_pscan = nullptr;
     int4 *d_work = nullptr;
-    int *d_qpos = nullptr, *d_qbegin = nullptr, *d_qend = nullptr;
+    int *d_qpos = nullptr, *d_qbegin = nullptr, *d_qend = nullptr;  // [2][NQ]: FAST items, GENERAL items
     size_t work_cap = 0, scan_cap = 0;
+    long long nkind[2] = {0, 0};  // work items of each tile kind (FAST first in d_work)
     DevHit *hits = nullptr;
     unsigned long long *nhits = nullptr;
     long long cap = 1ll << 22;
@@ -941,41 +942,41 @@ int build_batch_work(epi_batch *b, int form, const epi_schedule &sched)
     if (sched.order == 2)
         std::stable_sort(dorder.begin(), dorder.end(),
                          [&](int x, int y) { return b->data[x]->Kp > b->data[y]->Kp; });
+    // One list per tile kind (FAST items first in the work array), each run by its own persistent
+    // kernel so that a kernel holds the registers of one tile kind only; within a list the datasets
+    // follow the schedule's order. Each list is cut into its own queues.
     std::vector<int4> work;
-    std::vector<double> cost;
-    auto put = [&](int kind, int di) {
-        work.insert(work.end(), wk[kind][di].begin(), wk[kind][di].end());
-        cost.insert(cost.end(), ct[kind][di].begin(), ct[kind][di].end());
-    };
-    if (sched.order == 1) {
-        for (int di : dorder) put(1, di);
-        for (int di : dorder) put(0, di);
-    } else {
+    std::vector<int> qbegin(2 * NQ), qend(2 * NQ);
+    for (int kind = 0; kind < 2; kind++) {
+        const int base = (int)work.size();
+        std::vector<double> cost;
         for (int di : dorder) {
-            put(0, di);
-            put(1, di);
+            work.insert(work.end(), wk[kind][di].begin(), wk[kind][di].end());
+            cost.insert(cost.end(), ct[kind][di].begin(), ct[kind][di].end());
         }
-    }
-    b->nwork = (long long)work.size();
-    if (work.size() > (size_t)INT32_MAX / 2) return fail(c, EPI_ERR_RANGE, "batch has too many tiles");
-    const int n = (int)work.size();
-    std::vector<int> qbegin(NQ, n), qend(NQ, n);
-    qbegin[0] = 0;
-    if (sched.queues == 1) {
-        qend[0] = n;
-    } else {
-        double total = 0.0;
-        for (double x : cost) total += x;
-        double acc = 0.0;
-        int q = 0;
-        for (int i = 0; i < n && q < NQ - 1; i++) {
-            acc += cost[i];
-            if (acc >= total * (q + 1) / NQ) {
-                qend[q] = i + 1;
-                qbegin[++q] = i + 1;
+        if (work.size() > (size_t)INT32_MAX / 2) return fail(c, EPI_ERR_RANGE, "batch has too many tiles");
+        const int n = (int)work.size();
+        b->nkind[kind] = n - base;
+        int *qb = qbegin.data() + kind * NQ, *qe = qend.data() + kind * NQ;
+        for (int q = 0; q < NQ; q++) qb[q] = qe[q] = n;
+        qb[0] = base;
+        if (sched.queues == 1) {
+            qe[0] = n;
+        } else {
+            double total = 0.0;
+            for (double x : cost) total += x;
+            double acc = 0.0;
+            int q = 0;
+            for (int i = base; i < n && q < NQ - 1; i++) {
+                acc += cost[i - base];
+                if (acc >= total * (q + 1) / NQ) {
+                    qe[q] = i + 1;
+                    qb[++q] = i + 1;
+                }
             }
         }
     }
+    b->nwork = (long long)work.size();
     int rc;
     if (work.size() > b->work_cap) {
         if ((rc = dalloc(c, b->d_work, work.size()))) return rc;
@@ -984,8 +985,8 @@ int build_batch_work(epi_batch *b, int form, const epi_schedule &sched)
     if (!work.empty())
         HIPCHK(c, hipMemcpyAsync(b->d_work, work.data(), work.size() * sizeof(int4), hipMemcpyHostToDevice,
                                  c->stream));
-    HIPCHK(c, hipMemcpyAsync(b->d_qbegin, qbegin.data(), NQ * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(b->d_qend, qend.data(), NQ * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b->d_qbegin, qbegin.data(), 2 * NQ * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b->d_qend, qend.data(), 2 * NQ * sizeof(int), hipMemcpyHostToDevice, c->stream));
     // scan descriptors
     size_t ns = b->scans.size();
     if (ns > b->scan_cap) {
@@ -1022,44 +1023,59 @@ void drop_graph(epi_batch *b)
     b->gvalid = false;
 }
 
-const void *batch_kernel(int form, const epi_schedule &sc)
+const void *batch_kernel(int form, const epi_schedule &sc, int kind)
 {
-    if (form == EPI_FORM_MFMA)
-        return sc.fast_rows == 64 ? (const void *)mfma_batch_kernel<64> : (const void *)mfma_batch_kernel<32>;
-    return sc.fast_rows == 32 ? (const void *)plain_batch_kernel<2> : (const void *)plain_batch_kernel<1>;
+    if (form == EPI_FORM_MFMA) {
+        if (sc.fast_rows == 64)
+            return kind ? (const void *)mfma_batch_kernel<64, 1> : (const void *)mfma_batch_kernel<64, 0>;
+        return kind ? (const void *)mfma_batch_kernel<32, 1> : (const void *)mfma_batch_kernel<32, 0>;
+    }
+    if (sc.fast_rows == 32)
+        return kind ? (const void *)plain_batch_kernel<2, 1> : (const void *)plain_batch_kernel<2, 0>;
+    return kind ? (const void *)plain_batch_kernel<1, 1> : (const void *)plain_batch_kernel<1, 0>;
 }
 
-// Capture queue reset + persistent scan kernel as one graph. The grid is the
-// kernel's occupancy times the CU count: every workgroup is resident and
-// pulls work until the queues run dry.
+// Capture queue reset + the persistent scan kernels as one graph: one kernel
+// per tile kind that has items (GENERAL first under order 1), each with a grid
+// of its occupancy times the CU count, so that every workgroup is resident and
+// pulls work until its queues run dry.
 int capture_batch(epi_batch *b, const epi_scan_params *p)
 {
     epi_ctx *c = b->ctx;
     drop_graph(b);
-    BatchArgs B;
-    B.mscan = b->d_mscan;
-    B.pscan = b->d_pscan;
-    B.work = b->d_work;
-    B.qpos = b->d_qpos;
-    B.qend = b->d_qend;
-    B.hs = make_sink(b->hits, b->nhits, b->cap, p);
-    const void *kfn = batch_kernel(p->form, b->work_sched);
+    BatchArgs B[2];
+    const void *kfn[2];
+    int grid[2] = {0, 0};
     const int nt = p->form == EPI_FORM_MFMA ? 8 * b->work_sched.fast_rows : 256;
-    int per_cu = 1;
-    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, nt, 0));
-    const int grid = std::max(1, per_cu) * c->ncu;
-    void *args[] = {&B};
+    for (int kind = 0; kind < 2; kind++) {
+        B[kind].mscan = b->d_mscan;
+        B[kind].pscan = b->d_pscan;
+        B[kind].work = b->d_work;
+        B[kind].qpos = b->d_qpos + kind * NQ;
+        B[kind].qend = b->d_qend + kind * NQ;
+        B[kind].hs = make_sink(b->hits, b->nhits, b->cap, p);
+        kfn[kind] = batch_kernel(p->form, b->work_sched, kind);
+        int per_cu = 1;
+        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn[kind], nt, 0));
+        grid[kind] = std::max(1, per_cu) * c->ncu;
+    }
+    const int first = b->work_sched.order == 1 ? 1 : 0;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, c->stream, b->d_qpos, (const int *)b->d_qbegin, b->nhits);
     hipError_t le = hipGetLastError();
-    if (le == hipSuccess) le = hipLaunchKernel(kfn, dim3(grid), dim3(nt), args, 0, c->stream);
+    for (int t = 0; t < 2 && le == hipSuccess; t++) {
+        const int kind = t == 0 ? first : 1 - first;
+        if (b->nkind[kind] == 0) continue;
+        void *args[] = {&B[kind]};
+        le = hipLaunchKernel(kfn[kind], dim3(grid[kind]), dim3(nt), args, 0, c->stream);
+    }
     hipError_t ce = hipStreamEndCapture(c->stream, &b->graph);
     if (le != hipSuccess) return fail(c, EPI_ERR_HIP, "launch during capture: %s", hipGetErrorString(le));
     if (ce != hipSuccess) return fail(c, EPI_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ce));
     HIPCHK(c, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     b->gkey = *p;
     b->gvalid = true;
-    b->grid = grid;
+    b->grid = std::max(b->nkind[0] ? grid[0] : 0, b->nkind[1] ? grid[1] : 0);
     b->captures++;
     return EPI_OK;
 }
@@ -1640,7 +1656,8 @@ int epi_batch_create(epi_ctx *c, epi_batch **out)
     b->ctx = c;
     for (int f = 0; f < 2; f++) b->sched[f] = default_schedule(f);
     int rc;
-    if ((rc = dalloc(c, b->d_qpos, NQ)) || (rc = dalloc(c, b->d_qbegin, NQ)) || (rc = dalloc(c, b->d_qend, NQ)) ||
+    if ((rc = dalloc(c, b->d_qpos, 2 * NQ)) || (rc = dalloc(c, b->d_qbegin, 2 * NQ)) ||
+        (rc = dalloc(c, b->d_qend, 2 * NQ)) ||
         (rc = dalloc(c, b->nhits, 1)) || (rc = dalloc(c, b->hits, (size_t)b->cap))) {
         epi_batch_destroy(b);
         return rc;

@@ -1,7 +1,9 @@
 // Persistent batched scan: one launch walks the tiles of many scans (datasets
 // of uneven N and M, and several phenotypes per dataset).
 //
-// The host orders the work items so that consecutive items share SNP data
+// The host keeps one work list per tile kind (FAST, GENERAL), each run by its
+// own persistent kernel so that a kernel holds the registers of one tile kind
+// only. It orders the items of a list so that consecutive items share SNP data
 // (tile-major, phenotype-minor) and cuts the list into NQ contiguous queues of
 // equal estimated cost, one per XCD. A workgroup pulls items from the queue of
 // its XCD (workgroups are dealt to XCDs round-robin, so blockIdx % 8 is its
@@ -25,15 +27,16 @@ struct BatchArgs {
     const MfmaScan *mscan;   // [nscan]
     const PlainScan *pscan;  // [nscan]
     const int4 *work;        // {a0, b0, alen | blen << 8 | general << 16, scan}
-    int *qpos;               // [NQ] next item of each queue (reset to qbegin)
+    int *qpos;               // [NQ] next item of each queue of this kernel's list (reset to qbegin)
     const int *qend;         // [NQ] one past the last item of each queue
     HitSink hs;
 };
 
+// Queue heads of both lists ([2][NQ]) and the hit counter.
 __global__ void batch_reset(int *qpos, const int *qbegin, unsigned long long *nhits)
 {
     const int t = threadIdx.x;
-    if (t < NQ) qpos[t] = qbegin[t];
+    if (t < 2 * NQ) qpos[t] = qbegin[t];
     if (t == 0) *nhits = 0ull;
 }
 
@@ -51,13 +54,13 @@ __device__ __forceinline__ int next_item(const BatchArgs &B, int home)
 }
 
 // TA: FAST tiles are TA x 64 (TA = 32 or 64), GENERAL tiles TA x 32; a
-// workgroup has 8 TA threads for both.
-template <int TA>
+// workgroup has 8 TA threads for both. GEN: the kernel of the GENERAL list.
+template <int TA, int GEN>
 __global__ __launch_bounds__(fast_threads<TA>(), 64 / TA) void mfma_batch_kernel(const BatchArgs B)
 {
     constexpr int GA = TA;
     static_assert(general_threads<GA>() == fast_threads<TA>(), "one workgroup size for both tile kinds");
-    constexpr int LDS = fast_lds<TA>() > general_lds<GA>() ? fast_lds<TA>() : general_lds<GA>();
+    constexpr int LDS = GEN ? general_lds<GA>() : fast_lds<TA>();
     __shared__ __attribute__((aligned(16))) uint8_t lds[LDS];
     __shared__ int item;
     __shared__ int4 wsh;
@@ -87,15 +90,15 @@ __global__ __launch_bounds__(fast_threads<TA>(), 64 / TA) void mfma_batch_kernel
         HitSink hs = B.hs;
         hs.tag = sc;
         const int a1 = a0 + (ext & 255), b1 = b0 + ((ext >> 8) & 255);
-        if ((ext >> 16) == 0)
-            mfma_fast_tile<TA, false>(S, hs, nodump, a0, a1, b0, b1, lds);
-        else
+        if (GEN)
             mfma_general_tile<GA, false>(S, hs, nodump, a0, a1, b0, b1, lds);
+        else
+            mfma_fast_tile<TA, false>(S, hs, nodump, a0, a1, b0, b1, lds);
     }
 }
 
 // PA: FAST tiles are 16 PA x 64 (PA = 2 or 1); GENERAL tiles are 16 x 64.
-template <int PA>
+template <int PA, int GEN>
 __global__ __launch_bounds__(256) void plain_batch_kernel(const BatchArgs B)
 {
     __shared__ int item;
@@ -124,10 +127,10 @@ __global__ __launch_bounds__(256) void plain_batch_kernel(const BatchArgs B)
         HitSink hs = B.hs;
         hs.tag = sc;
         const int a1 = a0 + (ext & 255), b1 = b0 + ((ext >> 8) & 255);
-        if ((ext >> 16) == 0)
-            plain_tile<PA, 4, true, false>(S, hs, nodump, a0, a1, b0, b1);
-        else
+        if (GEN)
             plain_tile<1, 4, false, false>(S, hs, nodump, a0, a1, b0, b1);
+        else
+            plain_tile<PA, 4, true, false>(S, hs, nodump, a0, a1, b0, b1);
     }
 }
 
