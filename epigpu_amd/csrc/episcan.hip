@@ -12,6 +12,11 @@
 //     vbits [Kw]    uint32 phenotype-present bits
 //     marg  [nsnp]  Marg   per-SNP class counts and sums
 //     oidx, ocodes, ov     outlier individuals (at most OUT_MAX), their codes per SNP and values
+//     mp    [Kp]    MissPhen V, q, z per individual (SPARSE tiles)
+//   for SPARSE tiles (built at the first scan that has them, for the chromosome layout it was built for):
+//     moff  [nsnp+1] int64, mlist int32   per SNP, the individuals with a missing call
+//     vidx  [nsnp]  int32  virtual SNP index: each chromosome starts at a multiple of 64
+//     tbits [nw][nid][4] uint64  transposed bitplanes over the virtual order (64 SNPs per word)
 // Kp = nid rounded up to 256 (the plain kernel's stage), Kw = Kp / 32; padding
 // individuals are absent. The MFMA K loops stop at nid rounded up to 128.
 #include <hip/hip_runtime.h>
@@ -61,6 +66,7 @@ struct PhenDev {
     int32_t *oidx = nullptr;    // [nout] outlier individuals
     uint32_t *ocodes = nullptr; // [ceil(nout/16)][nsnp], built with the marginals
     OutVals *ov = nullptr;
+    MissPhen *mp = nullptr;
     int nout = 0;
     int fbits = 0;  // q = round((y - centre) * 2^fbits)
     int zbits = 0;  // Z limbs hold round(q^2 / 2^zbits)
@@ -84,16 +90,28 @@ struct Dataset {
     uint32_t *bits = nullptr;
     int32_t *dmiss = nullptr;
     int32_t *cls = nullptr;
-    std::vector<int> miss;   // per SNP: any missing call
-    std::vector<int> missc;  // prefix count of SNPs with missing calls
+    std::vector<int> miss;    // per SNP: missing calls
+    std::vector<int> missc;   // prefix count of SNPs with missing calls
+    std::vector<int> heavyc;  // prefix count of SNPs with more than sparse_max() missing calls
+    std::vector<long long> missum;  // prefix sum of missing calls
     std::vector<PhenDev> ph;
     bool marg_ok = false;
     std::vector<int> chrstart, chrsize, snpchr;
+    // SPARSE tiles
+    bool sp_ok = false;       // built for the current genotypes and chromosomes
+    long long *moff = nullptr;
+    int32_t *mlist = nullptr;
+    int32_t *vidx = nullptr;
+    uint64_t *tbits = nullptr;
+    long long tstride = 0;    // uint64 words per 64-SNP column of tbits (4 nid)
 };
 
+// Tile lists of a scan by kind: FAST (no missing call), SPARSE (missing
+// calls, few per SNP), GENERAL (the rest). Device order: FAST, SPARSE, GENERAL.
 struct TileLists {
-    std::vector<int4> fast, gen;
+    std::vector<int4> fast, gen, sp;
     long long pairs = 0;
+    size_t count() const { return fast.size() + sp.size() + gen.size(); }
 };
 
 }  // namespace
@@ -124,6 +142,7 @@ struct epi_ctx {
     double last_ms = 0.0;
     long long last_pairs = 0;
     int last_launches = 0;
+    long long last_tiles[3] = {0, 0, 0};  // FAST, SPARSE, GENERAL tiles of the last scan
     epi_schedule sched[2];  // per form
     void *scratch = nullptr;  // this device's no-allocation scratch (address from hipGetSymbolAddress)
 };
@@ -135,9 +154,9 @@ struct epi_batch {
     MfmaScan *d_mscan = nullptr;
     PlainScan *d_pscan = nullptr;
     int4 *d_work = nullptr;
-    int *d_qpos = nullptr, *d_qbegin = nullptr, *d_qend = nullptr;  // [2][NQ]: FAST items, GENERAL items
+    int *d_qpos = nullptr, *d_qbegin = nullptr, *d_qend = nullptr;  // [NKIND][NQ]: FAST, GENERAL, SPARSE items
     size_t work_cap = 0, scan_cap = 0;
-    long long nkind[2] = {0, 0};  // work items of each tile kind (FAST first in d_work)
+    long long nkind[NKIND] = {0, 0, 0};  // work items of each tile kind (FAST first in d_work)
     DevHit *hits = nullptr;
     unsigned long long *nhits = nullptr;
     long long cap = 1ll << 22;
@@ -215,6 +234,16 @@ void free_phen(PhenDev &p)
     dfree(p.oidx);
     dfree(p.ocodes);
     dfree(p.ov);
+    dfree(p.mp);
+}
+
+void drop_sparse(Dataset &d)
+{
+    dfree(d.moff);
+    dfree(d.mlist);
+    dfree(d.vidx);
+    dfree(d.tbits);
+    d.sp_ok = false;
 }
 
 void drop_phenotypes(Dataset &d)
@@ -231,15 +260,28 @@ void free_dataset(Dataset &d)
     dfree(d.bits);
     dfree(d.dmiss);
     dfree(d.cls);
+    drop_sparse(d);
     for (auto &p : d.ph) free_phen(p);
     d.ph.clear();
 }
 
 // ------------------------------------------------------ dataset preparation --
 
+// SNPs with at most this many missing calls can be in SPARSE tiles (the
+// per-pair counts of the missing-call sums are 16-bit).
+int sparse_max(int nid) { return std::min(65535, std::max(8, nid / 16)); }
+
+// A tile with missing calls is SPARSE (under dense_missing 0) when its SNPs
+// miss at most 1 / SPARSE_RATE of their calls on average: measured at
+// N = 4000 with 64-row tiles, the per-individual sums of a SPARSE tile cost
+// less than the NM rows of a GENERAL tile up to about 0.7 % missing calls
+// (0.5 %: 17.7 ms against 20.5 ms for the scan; 1 %: 24.7 against 20.5).
+constexpr long long SPARSE_RATE = 200;
+
 int ds_upload_genotypes(epi_ctx *c, Dataset &d, const int32_t *words, int nid, int nsnp)
 {
     if (!d.ph.empty() && nid != d.nid) drop_phenotypes(d);  // a new cohort: the old phenotypes no longer apply
+    drop_sparse(d);
     d.nid = nid;
     d.nsnp = nsnp;
     d.npack = nid / 16;
@@ -271,7 +313,14 @@ int ds_upload_genotypes(epi_ctx *c, Dataset &d, const int32_t *words, int nid, i
     HIPCHK(c, hipStreamSynchronize(c->stream));
     dfree(d.words);  // the uploaded words only feed the transcode
     d.missc.assign(nsnp + 1, 0);
-    for (int s = 0; s < nsnp; s++) d.missc[s + 1] = d.missc[s] + (d.miss[s] != 0);
+    d.heavyc.assign(nsnp + 1, 0);
+    d.missum.assign(nsnp + 1, 0);
+    const int smax = sparse_max(nid);
+    for (int s = 0; s < nsnp; s++) {
+        d.missc[s + 1] = d.missc[s] + (d.miss[s] != 0);
+        d.heavyc[s + 1] = d.heavyc[s] + (d.miss[s] > smax);
+        d.missum[s + 1] = d.missum[s] + d.miss[s];
+    }
     d.marg_ok = false;
     // chromosome layout resets to one block over all SNPs
     d.chrstart.assign(1, 0);
@@ -313,6 +362,7 @@ struct PhenEnc {
     std::vector<float> yc;      // [Kp]
     std::vector<uint32_t> vb;   // [Kw]
     std::vector<int32_t> oidx;  // outlier individuals
+    std::vector<MissPhen> mp;   // [Kp]
     OutVals ov = {};
     PhenDev p;                  // scalars only
 };
@@ -401,6 +451,7 @@ int encode_phen(epi_ctx *c, const float *y, int nid, int Kp, bool nine_value, in
     e.qv.assign(Kp, 0);
     e.yc.assign(Kp, 0.0f);
     e.vb.assign(Kw, 0u);
+    e.mp.assign(Kp, MissPhen{});
     long long qmax2 = 0;
     for (int k : P) {
         const long long q = out[k] ? 0 : std::llround(std::ldexp((double)y[k] - centre, f));
@@ -421,6 +472,12 @@ int encode_phen(epi_ctx *c, const float *y, int nid, int Kp, bool nine_value, in
         e.w[0 * (size_t)Kp + k] = 1;
         for (int t = 0; t < 3; t++) e.w[(1 + t) * (size_t)Kp + k] = lq[t];
         for (int t = 0; t < 4; t++) e.w[(4 + t) * (size_t)Kp + k] = lz[t];
+        MissPhen &mk = e.mp[k];
+        mk.v = 1;
+        mk.q = (int32_t)q;
+        mk.z = (int32_t)z;
+        mk.qf = (float)q;
+        mk.zf = (float)z;
         qsum += q;
         sq += (double)q * (double)q;
         const double dv = (double)y[k] - centre;
@@ -473,7 +530,7 @@ int ds_set_phenotypes(epi_ctx *c, Dataset &d, const float *phen, int nphen, int 
         p = e.p;
         if ((rc = dalloc(c, p.wts, e.w.size())) || (rc = dalloc(c, p.qv, (size_t)Kp)) ||
             (rc = dalloc(c, p.yc, (size_t)Kp)) || (rc = dalloc(c, p.vbits, (size_t)Kw)) ||
-            (rc = dalloc(c, p.ov, 1)))
+            (rc = dalloc(c, p.ov, 1)) || (rc = dalloc(c, p.mp, (size_t)Kp)))
             break;
         if (p.nout && (rc = dalloc(c, p.oidx, (size_t)p.nout))) break;
         hipError_t he = hipMemcpyAsync(p.wts, e.w.data(), e.w.size(), hipMemcpyHostToDevice, c->stream);
@@ -482,6 +539,8 @@ int ds_set_phenotypes(epi_ctx *c, Dataset &d, const float *phen, int nphen, int 
         if (he == hipSuccess)
             he = hipMemcpyAsync(p.vbits, e.vb.data(), (size_t)Kw * 4, hipMemcpyHostToDevice, c->stream);
         if (he == hipSuccess) he = hipMemcpyAsync(p.ov, &e.ov, sizeof(OutVals), hipMemcpyHostToDevice, c->stream);
+        if (he == hipSuccess)
+            he = hipMemcpyAsync(p.mp, e.mp.data(), (size_t)Kp * sizeof(MissPhen), hipMemcpyHostToDevice, c->stream);
         if (he == hipSuccess && p.nout)
             he = hipMemcpyAsync(p.oidx, e.oidx.data(), (size_t)p.nout * 4, hipMemcpyHostToDevice, c->stream);
         if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
@@ -517,6 +576,7 @@ int ds_set_chromosomes(epi_ctx *c, Dataset &d, const int32_t *chrstart, const in
     d.chrstart.assign(chrstart, chrstart + nchr);
     d.chrsize.assign(chrsize, chrsize + nchr);
     d.snpchr.swap(snpchr);
+    drop_sparse(d);  // its virtual SNP order follows the chromosomes
     return EPI_OK;
 }
 
@@ -532,7 +592,7 @@ int ds_marginals(epi_ctx *c, Dataset &d)
         if (rc) return rc;
         hipLaunchKernelGGL(marginals_kernel, dim3((d.nsnp + 63) / 64), dim3(256), 0, c->stream,
                            (const uint32_t *)d.mbits, d.Kw, d.nsnp, (const uint32_t *)p.vbits, (const int32_t *)p.qv,
-                           (const float *)p.yc, (const int32_t *)d.cls, p.marg);
+                           (const float *)p.yc, (const int32_t *)d.cls, (const MissPhen *)p.mp, p.marg);
         HIPCHK(c, hipGetLastError());
         if (p.nout) {
             if ((rc = dalloc(c, p.ocodes, (size_t)(p.nout + 15) / 16 * d.nsnp))) return rc;
@@ -544,6 +604,82 @@ int ds_marginals(epi_ctx *c, Dataset &d)
     HIPCHK(c, hipStreamSynchronize(c->stream));
     d.marg_ok = true;
     return EPI_OK;
+}
+
+// Virtual SNP order of the SPARSE tiles: chromosome i starts at a multiple of
+// 64, so that the a- or b-range of every tile (which starts at its
+// chromosome's start plus a multiple of the tile height) is one 64-SNP word,
+// or half of one for 32-row tiles. Returns the number of words.
+long long virtual_order(const Dataset &d, std::vector<int32_t> &vidx, std::vector<int32_t> *snp_of_v)
+{
+    vidx.assign(d.nsnp, 0);
+    long long nv = 0;
+    for (size_t i = 0; i < d.chrstart.size(); i++) {
+        for (int t = 0; t < d.chrsize[i]; t++) vidx[d.chrstart[i] + t] = (int32_t)(nv + t);
+        nv += (d.chrsize[i] + 63) / 64 * 64;
+    }
+    if (nv == 0) nv = 64;
+    if (snp_of_v) {
+        snp_of_v->assign(nv, -1);
+        for (size_t i = 0; i < d.chrstart.size(); i++)
+            for (int t = 0; t < d.chrsize[i]; t++) (*snp_of_v)[vidx[d.chrstart[i] + t]] = d.chrstart[i] + t;
+    }
+    return nv / 64;
+}
+
+// Device data of the SPARSE tiles for the current genotypes and chromosomes
+// (the miss lists, the virtual SNP order and the transposed bitplanes), built
+// when the genotypes or the chromosomes are set, for datasets with missing
+// calls.
+int ds_sparse(epi_ctx *c, Dataset &d)
+{
+    drop_sparse(d);
+    if (!d.mbits || d.missc.empty() || d.missc[d.nsnp] == 0) return EPI_OK;
+    std::vector<int32_t> vidx, sov;
+    const long long nw = virtual_order(d, vidx, &sov);
+    std::vector<long long> moff(d.nsnp + 1, 0);
+    for (int s = 0; s < d.nsnp; s++) moff[s + 1] = moff[s] + d.miss[s];
+    d.tstride = 4LL * d.nid;
+    int32_t *dsov = nullptr;
+    int rc;
+    if ((rc = dalloc(c, d.moff, (size_t)d.nsnp + 1)) || (rc = dalloc(c, d.mlist, (size_t)std::max(1LL, moff[d.nsnp]))) ||
+        (rc = dalloc(c, d.vidx, (size_t)d.nsnp)) || (rc = dalloc(c, d.tbits, (size_t)nw * d.tstride)) ||
+        (rc = dalloc(c, dsov, sov.size()))) {
+        dfree(dsov);
+        drop_sparse(d);
+        return rc;
+    }
+    hipError_t he = hipMemcpyAsync(d.moff, moff.data(), moff.size() * sizeof(long long), hipMemcpyHostToDevice,
+                                   c->stream);
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(d.vidx, vidx.data(), vidx.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream);
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(dsov, sov.data(), sov.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream);
+    if (he == hipSuccess) {
+        hipLaunchKernelGGL(miss_list_kernel, dim3((d.nsnp + 255) / 256), dim3(256), 0, c->stream,
+                           (const uint32_t *)d.mbits, d.Kw, d.nsnp, d.nid, (const long long *)d.moff, d.mlist);
+        he = hipGetLastError();
+    }
+    if (he == hipSuccess) {
+        hipLaunchKernelGGL(tbits_kernel, dim3((unsigned)nw, (d.nid + 31) / 32), dim3(64), 0, c->stream,
+                           (const uint32_t *)d.mbits, d.Kw, d.nsnp, d.nid, (const int32_t *)dsov, (int)nw, d.tbits);
+        he = hipGetLastError();
+    }
+    if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+    dfree(dsov);
+    if (he != hipSuccess) {
+        drop_sparse(d);
+        return fail(c, EPI_ERR_HIP, "SPARSE tile data: %s", hipGetErrorString(he));
+    }
+    d.sp_ok = true;
+    return EPI_OK;
+}
+
+// SPARSE tiles serve MFMA scans of datasets whose SPARSE data are built; the
+// schedule can turn them off.
+bool use_sparse(const Dataset &d, int form, const epi_schedule &sc)
+{
+    return form == EPI_FORM_MFMA && sc.dense_missing != 1 && d.sp_ok;
 }
 
 // MFMA scans of phenotype r need a phenotype the fixed point holds to 1e-5.
@@ -574,6 +710,12 @@ MfmaScan mfma_scan(const Dataset &d, int r, int tag)
     S.nout = p.nout;
     S.ouSum = p.ouSum;
     S.ouSq = p.ouSq;
+    S.tbits = d.tbits;
+    S.vidx = d.vidx;
+    S.moff = d.moff;
+    S.mlist = d.mlist;
+    S.mp = p.mp;
+    S.tstride = d.tstride;
     return S;
 }
 
@@ -654,21 +796,30 @@ bool valid_schedule(int form, const epi_schedule &s)
     bool rows = form == EPI_FORM_MFMA ? (s.fast_rows == 64 || s.fast_rows == 32)
                                       : (s.fast_rows == 32 || s.fast_rows == 16);
     return rows && s.super_a >= 1 && s.super_a <= 64 && s.super_b >= 1 && s.super_b <= 64 &&
-           (s.queues == 1 || s.queues == 8) && s.order >= 0 && s.order <= 2;
+           (s.queues == 1 || s.queues == 8) && s.order >= 0 && s.order <= 2 &&
+           (s.dense_missing >= 0 && s.dense_missing <= 2);
 }
 
 bool same_schedule(const epi_schedule &x, const epi_schedule &y)
 {
     return x.fast_rows == y.fast_rows && x.super_a == y.super_a && x.super_b == y.super_b && x.queues == y.queues &&
-           x.order == y.order;
+           x.order == y.order && x.dense_missing == y.dense_missing;
 }
 
-void emit_tile(const Dataset &d, int a0, int a1, int b0, int b1, int grows, int gcols, TileLists &out)
+void emit_tile(const Dataset &d, int a0, int a1, int b0, int b1, int grows, int gcols, bool sparse, bool sparse_all,
+               TileLists &out)
 {
     auto anymiss = [&](int s0, int s1) { return d.missc[s1] - d.missc[s0] > 0; };
+    auto anyheavy = [&](int s0, int s1) { return d.heavyc[s1] - d.heavyc[s0] > 0; };
     if (a1 - 1 <= b0) return;  // no pair with a > b
     if (!anymiss(a0, a1) && !anymiss(b0, b1)) {
         out.fast.push_back(make_int4(a0, a1, b0, b1));
+        return;
+    }
+    const long long nmiss = d.missum[a1] - d.missum[a0] + d.missum[b1] - d.missum[b0];
+    if (sparse && !anyheavy(a0, a1) && !anyheavy(b0, b1) &&
+        (sparse_all || nmiss * SPARSE_RATE <= (long long)(a1 - a0 + b1 - b0) * d.nid)) {
+        out.sp.push_back(make_int4(a0, a1, b0, b1));
         return;
     }
     for (int x = a0; x < a1; x += grows) {
@@ -701,6 +852,7 @@ bool build_tiles(const Dataset &d, int blk_end, int form, const epi_schedule &sc
     const int TA = sc.fast_rows, TB = TILE_B, grows = gen_rows(form, sc), gcols = gen_cols(form, sc);
     const int nchr = (int)d.chrstart.size();
     const int SA = sc.super_a, SB = sc.super_b;
+    const bool sparse = use_sparse(d, form, sc), sparse_all = sc.dense_missing == 2;
     for (int i = 0; i < nchr; i++) {
         for (int j = 0; j <= i; j++) {
             int blk = i * (i + 1) / 2 + j;
@@ -725,15 +877,15 @@ bool build_tiles(const Dataset &d, int blk_end, int form, const epi_schedule &sc
                         for (int tb = sb; tb < std::min(ntb, sb + SB); tb++) {
                             int a0 = A0 + ta * TA, a1 = std::min(A1, a0 + TA);
                             int b0 = B0 + tb * TB, b1 = std::min(B1, b0 + TB);
-                            emit_tile(d, a0, a1, b0, b1, grows, gcols, out);
+                            emit_tile(d, a0, a1, b0, b1, grows, gcols, sparse, sparse_all, out);
                         }
-                if (out.fast.size() + out.gen.size() >= max_tiles && sa + SA < nta) {
+                if (out.count() >= max_tiles && sa + SA < nta) {
                     cur.sa = sa + SA;
                     return false;
                 }
             }
             cur.sa = 0;
-            if (out.fast.size() + out.gen.size() >= max_tiles) {
+            if (out.count() >= max_tiles) {
                 cur.blk = blk + 1;
                 return blk + 1 >= blk_end || blk + 1 >= nchr * (nchr + 1) / 2;
             }
@@ -742,22 +894,28 @@ bool build_tiles(const Dataset &d, int blk_end, int form, const epi_schedule &sc
     return true;
 }
 
+// The lists of one scan into dst, in device order (FAST, SPARSE, GENERAL).
+int copy_tiles(epi_ctx *c, int4 *dst, const TileLists &tl)
+{
+    size_t o = 0;
+    for (const std::vector<int4> *v : {&tl.fast, &tl.sp, &tl.gen}) {
+        if (!v->empty())
+            HIPCHK(c, hipMemcpyAsync(dst + o, v->data(), v->size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+        o += v->size();
+    }
+    return EPI_OK;
+}
+
 int upload_tiles(epi_ctx *c, const TileLists &tl)
 {
     c->tl_valid = false;
-    size_t need = tl.fast.size() + tl.gen.size();
+    size_t need = tl.count();
     if (need > c->tiles_cap) {
         int rc = dalloc(c, c->tiles, need);
         if (rc) return rc;
         c->tiles_cap = need;
     }
-    if (!tl.fast.empty())
-        HIPCHK(c, hipMemcpyAsync(c->tiles, tl.fast.data(), tl.fast.size() * sizeof(int4), hipMemcpyHostToDevice,
-                                 c->stream));
-    if (!tl.gen.empty())
-        HIPCHK(c, hipMemcpyAsync(c->tiles + tl.fast.size(), tl.gen.data(), tl.gen.size() * sizeof(int4),
-                                 hipMemcpyHostToDevice, c->stream));
-    return EPI_OK;
+    return copy_tiles(c, c->tiles, tl);
 }
 
 HitSink make_sink(DevHit *hits, unsigned long long *nhits, long long cap, const epi_scan_params *p)
@@ -828,22 +986,28 @@ int launch_pairs(epi_ctx *c, const epi_scan_params *p, const epi_schedule &sc, c
                  const DumpSink &ds, const ScanBufs &B)
 {
     const Dataset &d = c->d;
-    if (tl.fast.size() + tl.gen.size() > (size_t)INT32_MAX)
+    if (tl.count() > (size_t)INT32_MAX)
         return fail(c, EPI_ERR_RANGE, "a super-tile row of this scan exceeds 2^31 tiles");
-    const int nf = (int)tl.fast.size(), ng = (int)tl.gen.size();
+    const int nf = (int)tl.fast.size(), ns = (int)tl.sp.size(), ng = (int)tl.gen.size();
     c->last_launches = 0;
+    c->last_tiles[0] += nf;
+    c->last_tiles[1] += ns;
+    c->last_tiles[2] += ng;
     const HitSink hs = make_sink(B.hits, B.nhits, B.cap, p);
     if (p->form == EPI_FORM_MFMA) {
         MfmaArgs A;
         A.S = mfma_scan(d, p->phen, p->phen);
         A.hs = hs;
         A.ds = ds;
+        if (ns && !d.sp_ok) return fail(c, EPI_ERR_STATE, "SPARSE tile data not built");
         if (sc.fast_rows == 64) {
             LAUNCH_TILES((mfma_fast_kernel<64, DUMP>), fast_threads<64>(), A, B.tiles, nf);
-            LAUNCH_TILES((mfma_general_kernel<64, DUMP>), general_threads<64>(), A, B.tiles + nf, ng);
+            LAUNCH_TILES((mfma_sparse_kernel<64, DUMP>), fast_threads<64>(), A, B.tiles + nf, ns);
+            LAUNCH_TILES((mfma_general_kernel<64, DUMP>), general_threads<64>(), A, B.tiles + nf + ns, ng);
         } else {
             LAUNCH_TILES((mfma_fast_kernel<32, DUMP>), fast_threads<32>(), A, B.tiles, nf);
-            LAUNCH_TILES((mfma_general_kernel<32, DUMP>), general_threads<32>(), A, B.tiles + nf, ng);
+            LAUNCH_TILES((mfma_sparse_kernel<32, DUMP>), fast_threads<32>(), A, B.tiles + nf, ns);
+            LAUNCH_TILES((mfma_general_kernel<32, DUMP>), general_threads<32>(), A, B.tiles + nf + ns, ng);
         }
     } else {
         PlainArgs P;
@@ -854,7 +1018,7 @@ int launch_pairs(epi_ctx *c, const epi_scan_params *p, const epi_schedule &sc, c
             LAUNCH_TILES((plain_kernel<2, 4, true, DUMP>), 256, P, B.tiles, nf);
         else
             LAUNCH_TILES((plain_kernel<1, 4, true, DUMP>), 256, P, B.tiles, nf);
-        LAUNCH_TILES((plain_kernel<1, 4, false, DUMP>), 256, P, B.tiles + nf, ng);
+        LAUNCH_TILES((plain_kernel<1, 4, false, DUMP>), 256, P, B.tiles + nf + ns, ng);
     }
     return EPI_OK;
 }
@@ -898,23 +1062,24 @@ void emit_hits(const Dataset &d, const std::vector<DevHit> &h, const std::vector
 
 // Work items of every scan and the queue boundaries of the schedule.
 // Per dataset the items are in locality order: tile-major (super-tiles),
-// phenotype-minor, FAST tiles before GENERAL ones. order 1 puts the GENERAL
-// items of all datasets first, order 2 takes datasets by decreasing Kp, so
-// that the longest items do not end up in the tail. With 8 queues the list
+// phenotype-minor, one list per tile kind. order 1 runs the GENERAL and
+// SPARSE lists before the FAST one, order 2 takes datasets by decreasing Kp,
+// so that the longest items do not end up in the tail. With 8 queues the list
 // is cut into NQ contiguous pieces of equal estimated cost, one per XCD;
 // with 1 queue every workgroup pulls from one shared queue.
 int build_batch_work(epi_batch *b, int form, const epi_schedule &sched)
 {
     epi_ctx *c = b->ctx;
     const int nd = (int)b->data.size();
-    std::vector<std::vector<int4>> wk[2];
-    std::vector<std::vector<double>> ct[2];
-    for (int kind = 0; kind < 2; kind++) {
+    std::vector<std::vector<int4>> wk[NKIND];
+    std::vector<std::vector<double>> ct[NKIND];
+    for (int kind = 0; kind < NKIND; kind++) {
         wk[kind].resize(nd);
         ct[kind].resize(nd);
     }
     b->pairs = 0;
-    const double unit_fast = form == EPI_FORM_MFMA ? 16.0 : 4.0, unit_gen = form == EPI_FORM_MFMA ? 40.0 : 10.0;
+    // estimated cost per pair and individual, by kind
+    const double unit[NKIND] = {form == EPI_FORM_MFMA ? 16.0 : 4.0, form == EPI_FORM_MFMA ? 40.0 : 10.0, 22.0};
     for (int di = 0; di < nd; di++) {
         const Dataset &d = *b->data[di];
         std::vector<int> sc;
@@ -925,11 +1090,11 @@ int build_batch_work(epi_batch *b, int form, const epi_schedule &sched)
         TileCursor all;
         build_tiles(d, 1 << 30, form, sched, SIZE_MAX, all, tl);
         b->pairs += tl.pairs * (long long)sc.size();
-        for (int kind = 0; kind < 2; kind++) {
-            const std::vector<int4> &v = kind ? tl.gen : tl.fast;
+        for (int kind = 0; kind < NKIND; kind++) {
+            const std::vector<int4> &v = kind == 0 ? tl.fast : kind == 1 ? tl.gen : tl.sp;
             for (const int4 &t : v) {
                 int ext = (t.y - t.x) | ((t.w - t.z) << 8) | (kind << 16);
-                double cst = (double)(t.y - t.x) * (t.w - t.z) * d.Kp * (kind ? unit_gen : unit_fast) + 5e4;
+                double cst = (double)(t.y - t.x) * (t.w - t.z) * d.Kp * unit[kind] + 5e4;
                 for (int s : sc) {
                     wk[kind][di].push_back(make_int4(t.x, t.z, ext, s));
                     ct[kind][di].push_back(cst);
@@ -946,8 +1111,8 @@ int build_batch_work(epi_batch *b, int form, const epi_schedule &sched)
     // kernel so that a kernel holds the registers of one tile kind only; within a list the datasets
     // follow the schedule's order. Each list is cut into its own queues.
     std::vector<int4> work;
-    std::vector<int> qbegin(2 * NQ), qend(2 * NQ);
-    for (int kind = 0; kind < 2; kind++) {
+    std::vector<int> qbegin(NKIND * NQ), qend(NKIND * NQ);
+    for (int kind = 0; kind < NKIND; kind++) {
         const int base = (int)work.size();
         std::vector<double> cost;
         for (int di : dorder) {
@@ -985,8 +1150,9 @@ int build_batch_work(epi_batch *b, int form, const epi_schedule &sched)
     if (!work.empty())
         HIPCHK(c, hipMemcpyAsync(b->d_work, work.data(), work.size() * sizeof(int4), hipMemcpyHostToDevice,
                                  c->stream));
-    HIPCHK(c, hipMemcpyAsync(b->d_qbegin, qbegin.data(), 2 * NQ * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(b->d_qend, qend.data(), 2 * NQ * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b->d_qbegin, qbegin.data(), NKIND * NQ * sizeof(int), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(b->d_qend, qend.data(), NKIND * NQ * sizeof(int), hipMemcpyHostToDevice, c->stream));
     // scan descriptors
     size_t ns = b->scans.size();
     if (ns > b->scan_cap) {
@@ -1027,27 +1193,32 @@ const void *batch_kernel(int form, const epi_schedule &sc, int kind)
 {
     if (form == EPI_FORM_MFMA) {
         if (sc.fast_rows == 64)
-            return kind ? (const void *)mfma_batch_kernel<64, 1> : (const void *)mfma_batch_kernel<64, 0>;
-        return kind ? (const void *)mfma_batch_kernel<32, 1> : (const void *)mfma_batch_kernel<32, 0>;
+            return kind == 2 ? (const void *)mfma_batch_kernel<64, 2>
+                   : kind    ? (const void *)mfma_batch_kernel<64, 1>
+                             : (const void *)mfma_batch_kernel<64, 0>;
+        return kind == 2 ? (const void *)mfma_batch_kernel<32, 2>
+               : kind    ? (const void *)mfma_batch_kernel<32, 1>
+                         : (const void *)mfma_batch_kernel<32, 0>;
     }
+    // the plain form has no SPARSE tiles
     if (sc.fast_rows == 32)
         return kind ? (const void *)plain_batch_kernel<2, 1> : (const void *)plain_batch_kernel<2, 0>;
     return kind ? (const void *)plain_batch_kernel<1, 1> : (const void *)plain_batch_kernel<1, 0>;
 }
 
 // Capture queue reset + the persistent scan kernels as one graph: one kernel
-// per tile kind that has items (GENERAL first under order 1), each with a grid
+// per tile kind that has items (GENERAL and SPARSE first under order 1), each with a grid
 // of its occupancy times the CU count, so that every workgroup is resident and
 // pulls work until its queues run dry.
 int capture_batch(epi_batch *b, const epi_scan_params *p)
 {
     epi_ctx *c = b->ctx;
     drop_graph(b);
-    BatchArgs B[2];
-    const void *kfn[2];
-    int grid[2] = {0, 0};
+    BatchArgs B[NKIND];
+    const void *kfn[NKIND];
+    int grid[NKIND] = {0, 0, 0};
     const int nt = p->form == EPI_FORM_MFMA ? 8 * b->work_sched.fast_rows : 256;
-    for (int kind = 0; kind < 2; kind++) {
+    for (int kind = 0; kind < NKIND; kind++) {
         B[kind].mscan = b->d_mscan;
         B[kind].pscan = b->d_pscan;
         B[kind].work = b->d_work;
@@ -1059,12 +1230,13 @@ int capture_batch(epi_batch *b, const epi_scan_params *p)
         HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn[kind], nt, 0));
         grid[kind] = std::max(1, per_cu) * c->ncu;
     }
-    const int first = b->work_sched.order == 1 ? 1 : 0;
+    static const int korder[2][NKIND] = {{0, 2, 1}, {1, 2, 0}};
+    const int *ko = korder[b->work_sched.order == 1 ? 1 : 0];
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, c->stream, b->d_qpos, (const int *)b->d_qbegin, b->nhits);
     hipError_t le = hipGetLastError();
-    for (int t = 0; t < 2 && le == hipSuccess; t++) {
-        const int kind = t == 0 ? first : 1 - first;
+    for (int t = 0; t < NKIND && le == hipSuccess; t++) {
+        const int kind = ko[t];
         if (b->nkind[kind] == 0) continue;
         void *args[] = {&B[kind]};
         le = hipLaunchKernel(kfn[kind], dim3(grid[kind]), dim3(nt), args, 0, c->stream);
@@ -1075,7 +1247,9 @@ int capture_batch(epi_batch *b, const epi_scan_params *p)
     HIPCHK(c, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     b->gkey = *p;
     b->gvalid = true;
-    b->grid = std::max(b->nkind[0] ? grid[0] : 0, b->nkind[1] ? grid[1] : 0);
+    b->grid = 0;
+    for (int kind = 0; kind < NKIND; kind++)
+        if (b->nkind[kind]) b->grid = std::max(b->grid, grid[kind]);
     b->captures++;
     return EPI_OK;
 }
@@ -1171,6 +1345,7 @@ int run_scan(epi_ctx *c, const epi_scan_params *p, const epi_schedule &sc)
     memset(&ds, 0, sizeof(ds));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     long long pairs = tl.pairs;
+    c->last_tiles[0] = c->last_tiles[1] = c->last_tiles[2] = 0;
     int launches = 0;
     for (;;) {
         if ((rc = launch_pairs<false>(c, p, sc, tl, ds, ctx_bufs(c)))) return rc;
@@ -1235,6 +1410,12 @@ int tune_descent(int form, int dims, epi_schedule &cur, double &tcur, int &nmeas
     };
     const int rows_m[] = {32, 64}, rows_p[] = {32, 16};
     const int sup[][2] = {{8, 4}, {4, 4}, {16, 2}, {4, 8}, {1, 1}};
+    if ((dims & EPI_TUNE_MISSING) && form == EPI_FORM_MFMA)
+        for (int v : {0, 1, 2}) {
+            epi_schedule s = cur;
+            s.dense_missing = v;
+            if ((rc = tryit(s))) return rc;
+        }
     if (dims & EPI_TUNE_ROWS)
         for (int v : (form == EPI_FORM_MFMA ? rows_m : rows_p)) {
             epi_schedule s = cur;
@@ -1358,7 +1539,8 @@ int epi_set_genotypes(epi_ctx *c, const int32_t *words, int32_t nid, int32_t nsn
     HIPCHK(c, hipSetDevice(c->device));
     c->last_nhits = 0;
     c->tl_valid = false;
-    const int rc = ds_set_genotypes(c, c->d, words, nid, nsnp);
+    int rc = ds_set_genotypes(c, c->d, words, nid, nsnp);
+    if (!rc) rc = ds_sparse(c, c->d);
     return rc ? rc : ready_marginals(c);
 }
 
@@ -1399,8 +1581,11 @@ int epi_phen_encoding(const float *phen, int32_t nid, int32_t flags, epi_phen_in
 int epi_set_chromosomes(epi_ctx *c, const int32_t *chrstart, const int32_t *chrsize, int32_t nchr)
 {
     if (!c) return EPI_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
     c->tl_valid = false;
-    return ds_set_chromosomes(c, c->d, chrstart, chrsize, nchr);
+    int rc = ds_set_chromosomes(c, c->d, chrstart, chrsize, nchr);
+    if (!rc) rc = ds_sparse(c, c->d);
+    return rc;
 }
 
 void epi_scan_params_default(epi_scan_params *p)
@@ -1461,7 +1646,7 @@ int epi_scan_noalloc(epi_ctx *c, const epi_scan_params *p, epi_hit64 *hits, int6
     TileCursor cur;
     cur.blk = bb;
     const bool done = build_tiles(c->d, be, p->form, sc, NOALLOC_TILES + 1, cur, tl);
-    const size_t nt = tl.fast.size() + tl.gen.size();
+    const size_t nt = tl.count();
     if (!done || nt > NOALLOC_TILES)
         return fail(c, EPI_ERR_RANGE,
                     "the scan needs more than the %zu tiles the no-allocation scratch holds (%zu MiB); use epi_scan",
@@ -1469,12 +1654,8 @@ int epi_scan_noalloc(epi_ctx *c, const epi_scan_params *p, epi_hit64 *hits, int6
     unsigned char *base = static_cast<unsigned char *>(c->scratch);
     const ScanBufs B = {reinterpret_cast<int4 *>(base + 256), reinterpret_cast<DevHit *>(hits),
                         reinterpret_cast<unsigned long long *>(base), (long long)cap};
-    if (!tl.fast.empty())
-        HIPCHK(c, hipMemcpyAsync(B.tiles, tl.fast.data(), tl.fast.size() * sizeof(int4), hipMemcpyHostToDevice,
-                                 c->stream));
-    if (!tl.gen.empty())
-        HIPCHK(c, hipMemcpyAsync(B.tiles + tl.fast.size(), tl.gen.data(), tl.gen.size() * sizeof(int4),
-                                 hipMemcpyHostToDevice, c->stream));
+    if ((rc = copy_tiles(c, B.tiles, tl))) return rc;
+    c->last_tiles[0] = c->last_tiles[1] = c->last_tiles[2] = 0;
     hipLaunchKernelGGL(zero_counter, dim3(1), dim3(1), 0, c->stream, B.nhits);
     HIPCHK(c, hipGetLastError());
     DumpSink ds;
@@ -1500,8 +1681,10 @@ int epi_autotune(epi_ctx *c, const epi_scan_params *p, int32_t dims, int32_t rep
     int rc = check_params(c, p);
     if (rc) return rc;
     if (reps < 1 || reps > 100) return fail(c, EPI_ERR_ARG, "reps must be in [1, 100]");
-    if (dims == 0) dims = EPI_TUNE_ROWS | EPI_TUNE_SUPER;
-    dims &= EPI_TUNE_ROWS | EPI_TUNE_SUPER;  // queues and order do not apply to single scans
+    // queues and order do not apply to single scans; the missing-call tiles only with missing calls
+    const bool anymiss = !c->d.missc.empty() && c->d.missc[c->d.nsnp] > 0;
+    if (dims == 0) dims = EPI_TUNE_ROWS | EPI_TUNE_SUPER | (anymiss ? EPI_TUNE_MISSING : 0);
+    dims &= EPI_TUNE_ROWS | EPI_TUNE_SUPER | EPI_TUNE_MISSING;
     HIPCHK(c, hipSetDevice(c->device));
     if ((rc = ds_marginals(c, c->d))) return rc;
     epi_schedule cur = c->sched[p->form];
@@ -1577,6 +1760,15 @@ int epi_last_scan_stats(epi_ctx *c, double *kernel_ms, int64_t *pairs, int32_t *
     return EPI_OK;
 }
 
+int epi_last_tiles(epi_ctx *c, int64_t *fast, int64_t *sparse, int64_t *general)
+{
+    if (!c) return EPI_ERR_ARG;
+    if (fast) *fast = c->last_tiles[0];
+    if (sparse) *sparse = c->last_tiles[1];
+    if (general) *general = c->last_tiles[2];
+    return EPI_OK;
+}
+
 int epi_pair_dump(epi_ctx *c, const epi_scan_params *p, int32_t a0, int32_t a1, int32_t b0, int32_t b1,
                   int32_t *counts, double *sums, double *F, double *F2)
 {
@@ -1589,12 +1781,24 @@ int epi_pair_dump(epi_ctx *c, const epi_scan_params *p, int32_t a0, int32_t a1, 
     if ((rc = ds_marginals(c, c->d))) return rc;
     const epi_schedule &sc = c->sched[p->form];
     const int TA = sc.fast_rows, TB = TILE_B;
-    // Tiles over the rectangle in global SNP space (no block restriction).
+    // Tiles over the rectangle in global SNP space (no block restriction). A
+    // tile can be SPARSE where its a- and b-ranges each lie in one chromosome
+    // and start on the virtual word grid, as every tile of a scan does.
+    const Dataset &d = c->d;
+    const bool sp = use_sparse(d, p->form, sc);
+    std::vector<int32_t> vidx;
+    if (sp) virtual_order(d, vidx, nullptr);
+    auto grid = [&](int s0, int s1, int w) {
+        const int ch = d.snpchr[s0];
+        return ch >= 0 && d.snpchr[s1 - 1] == ch && vidx[s0] % w == 0;
+    };
     TileLists all;
     for (int x = a0; x < a1; x += TA)
-        for (int y = b0; y < b1; y += TB)
-            emit_tile(c->d, x, std::min(a1, x + TA), y, std::min(b1, y + TB), gen_rows(p->form, sc),
-                      gen_cols(p->form, sc), all);
+        for (int y = b0; y < b1; y += TB) {
+            const int x1 = std::min(a1, x + TA), y1 = std::min(b1, y + TB);
+            emit_tile(d, x, x1, y, y1, gen_rows(p->form, sc), gen_cols(p->form, sc),
+                      sp && grid(x, x1, TA) && grid(y, y1, 64), sc.dense_missing == 2, all);
+        }
     if ((rc = upload_tiles(c, all))) return rc;
     long long np = (long long)(a1 - a0) * (b1 - b0);
     int32_t *dc = nullptr;
@@ -1628,6 +1832,7 @@ int epi_pair_dump(epi_ctx *c, const epi_scan_params *p, int32_t a0, int32_t a1, 
     h2d(dsum, sums, (size_t)np * 9 * 8);
     h2d(dF, F, (size_t)np * 8);
     h2d(dF2, F2, (size_t)np * 8);
+    c->last_tiles[0] = c->last_tiles[1] = c->last_tiles[2] = 0;
     if (hrc == EPI_OK) hrc = launch_pairs<true>(c, p, sc, all, ds, ctx_bufs(c));
     auto d2h = [&](void *hdst, const void *d, size_t bytes) {
         if (hrc == EPI_OK && hdst && hipMemcpyAsync(hdst, d, bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
@@ -1656,8 +1861,8 @@ int epi_batch_create(epi_ctx *c, epi_batch **out)
     b->ctx = c;
     for (int f = 0; f < 2; f++) b->sched[f] = default_schedule(f);
     int rc;
-    if ((rc = dalloc(c, b->d_qpos, 2 * NQ)) || (rc = dalloc(c, b->d_qbegin, 2 * NQ)) ||
-        (rc = dalloc(c, b->d_qend, 2 * NQ)) ||
+    if ((rc = dalloc(c, b->d_qpos, NKIND * NQ)) || (rc = dalloc(c, b->d_qbegin, NKIND * NQ)) ||
+        (rc = dalloc(c, b->d_qend, NKIND * NQ)) ||
         (rc = dalloc(c, b->nhits, 1)) || (rc = dalloc(c, b->hits, (size_t)b->cap))) {
         epi_batch_destroy(b);
         return rc;
@@ -1697,6 +1902,7 @@ int epi_batch_add_dataset(epi_batch *b, const int32_t *words, int32_t nid, int32
     Dataset *d = new Dataset();
     int rc = ds_set_genotypes(c, *d, words, nid, nsnp);
     if (!rc && chrstart) rc = ds_set_chromosomes(c, *d, chrstart, chrsize, nchr);
+    if (!rc) rc = ds_sparse(c, *d);
     if (rc) {
         free_dataset(*d);
         delete d;

@@ -1,7 +1,7 @@
 // Persistent batched scan: one launch walks the tiles of many scans (datasets
 // of uneven N and M, and several phenotypes per dataset).
 //
-// The host keeps one work list per tile kind (FAST, GENERAL), each run by its
+// The host keeps one work list per tile kind (FAST, GENERAL, SPARSE), each run by its
 // own persistent kernel so that a kernel holds the registers of one tile kind
 // only. It orders the items of a list so that consecutive items share SNP data
 // (tile-major, phenotype-minor) and cuts the list into NQ contiguous queues of
@@ -21,7 +21,8 @@
 
 namespace epi {
 
-constexpr int NQ = 8;  // work queues, one per XCD
+constexpr int NQ = 8;     // work queues, one per XCD
+constexpr int NKIND = 3;  // tile kinds: 0 FAST, 1 GENERAL, 2 SPARSE
 
 struct BatchArgs {
     const MfmaScan *mscan;   // [nscan]
@@ -32,11 +33,11 @@ struct BatchArgs {
     HitSink hs;
 };
 
-// Queue heads of both lists ([2][NQ]) and the hit counter.
+// Queue heads of every list ([NKIND][NQ]) and the hit counter.
 __global__ void batch_reset(int *qpos, const int *qbegin, unsigned long long *nhits)
 {
     const int t = threadIdx.x;
-    if (t < 2 * NQ) qpos[t] = qbegin[t];
+    if (t < NKIND * NQ) qpos[t] = qbegin[t];
     if (t == 0) *nhits = 0ull;
 }
 
@@ -53,14 +54,15 @@ __device__ __forceinline__ int next_item(const BatchArgs &B, int home)
     return -1;
 }
 
-// TA: FAST tiles are TA x 64 (TA = 32 or 64), GENERAL tiles TA x 32; a
-// workgroup has 8 TA threads for both. GEN: the kernel of the GENERAL list.
-template <int TA, int GEN>
+// TA: FAST and SPARSE tiles are TA x 64 (TA = 32 or 64), GENERAL tiles
+// TA x 32; a workgroup has 8 TA threads for all three. KIND: the list this
+// kernel runs (0 FAST, 1 GENERAL, 2 SPARSE).
+template <int TA, int KIND>
 __global__ __launch_bounds__(fast_threads<TA>(), 64 / TA) void mfma_batch_kernel(const BatchArgs B)
 {
     constexpr int GA = TA;
-    static_assert(general_threads<GA>() == fast_threads<TA>(), "one workgroup size for both tile kinds");
-    constexpr int LDS = GEN ? general_lds<GA>() : fast_lds<TA>();
+    static_assert(general_threads<GA>() == fast_threads<TA>(), "one workgroup size for every tile kind");
+    constexpr int LDS = KIND == 1 ? general_lds<GA>() : KIND == 2 ? sparse_lds<TA>() : fast_lds<TA>();
     __shared__ __attribute__((aligned(16))) uint8_t lds[LDS];
     __shared__ int item;
     __shared__ int4 wsh;
@@ -90,8 +92,10 @@ __global__ __launch_bounds__(fast_threads<TA>(), 64 / TA) void mfma_batch_kernel
         HitSink hs = B.hs;
         hs.tag = sc;
         const int a1 = a0 + (ext & 255), b1 = b0 + ((ext >> 8) & 255);
-        if (GEN)
+        if (KIND == 1)
             mfma_general_tile<GA, false>(S, hs, nodump, a0, a1, b0, b1, lds);
+        else if (KIND == 2)
+            mfma_sparse_tile<TA, false>(S, hs, nodump, a0, a1, b0, b1, lds);
         else
             mfma_fast_tile<TA, false>(S, hs, nodump, a0, a1, b0, b1, lds);
     }

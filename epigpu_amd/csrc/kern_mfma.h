@@ -55,6 +55,16 @@ struct Marg {                    // per SNP, per phenotype, over present phenoty
     double ys[3];                // plain form: sum of FP32 centred phenotype of plane 0, 1, NM
     int32_t pn[3];               // plain form: count of plane 0, 1, NM
     int32_t pad;
+    long long zs;                // sum of z = round(q^2 / 2^zbits) over NM
+};
+
+struct MissPhen {                // per individual, for the SPARSE tiles' missing-call sums
+    int32_t v;                   // phenotype present (0 or 1)
+    float qf;                    // q as FP32
+    float zf;                    // z as FP32
+    int32_t q;                   // fixed-point phenotype (0 where missing and for outliers)
+    int32_t z;                   // round(q^2 / 2^zbits)
+    int32_t pad[3];
 };
 
 struct OutVals {                 // values of the outlier individuals, in fixed-point units
@@ -79,6 +89,14 @@ struct MfmaScan {                // one dataset x one phenotype
     double sqV;                  // sum q^2 over present (outliers excluded)
     double zunit;                // 2^zbits: q^2 units per unit of the Z limbs
     double ouSum, ouSq;          // sum of the outliers' values and of their squares
+    // SPARSE tiles (null without missing calls):
+    const uint64_t *tbits;       // [nw][nid][4]: bit v % 64 of word (v / 64, i, plane) = plane bit of the SNP
+                                 //   with virtual index v at individual i (planes: class 0, class 1, call present)
+    const int32_t *vidx;         // [nsnp] virtual index: chromosomes start at multiples of 64
+    const long long *moff;       // [nsnp + 1] offsets into mlist
+    const int32_t *mlist;        // per SNP, ascending: the individuals with a missing call
+    const MissPhen *mp;          // [Kp] for this phenotype
+    long long tstride;           // uint64 words per 64-SNP column of tbits (4 nid)
 };
 
 // Class of outlier 16w+t at a SNP from its code word w (3: no call).
@@ -407,9 +425,11 @@ __device__ __forceinline__ void k_loop1(St &sg, uint8_t *lds, int stage_bytes, i
 template <int TA>
 constexpr int fast_threads() { return 8 * TA; }
 
-template <int TA, bool DUMP>
-__device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink &hs, const DumpSink &dsk, int a0,
-                                               int a1, int b0, int b1, uint8_t *lds)
+// The K loop of a FAST tile: acc[i][2bg + h][8p + 4g + w] = X[g][h][w] of
+// the pair (a0 + 8 wid + 4i + 2hf + p, b0 + 32bg + m), m = lane & 31, hf = lane >> 5,
+// for classes g, h in {0, 1} and weights w = V, L0, L1, L2.
+template <int TA>
+__device__ __forceinline__ void fast_kloop(const MfmaScan &A, int a0, int b0, uint8_t *lds, v16i (&acc)[2][4])
 {
     static_assert(TA == 64 || TA == 32, "fast tile height");
     constexpr int TB = 64, NPL = 2, NT = fast_threads<TA>();
@@ -437,7 +457,6 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
     const int offB = (NPL * TA + m) * ROWB + 16 * hf;          // + (h*TB + 32*bg) * ROWB
     const int offA = (ag * TA + APW * wid + aj) * ROWB + 16 * hf; // + 4*i*ROWB
 
-    v16i acc[MB][4];
 #pragma unroll
     for (int i = 0; i < MB; i++)
 #pragma unroll
@@ -467,6 +486,17 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
     };
 
     k_loop<decltype(sg), Ops, 2 * 4>(sg, lds, STAGE, A.Kl, fetch, issue);
+}
+
+template <int TA, bool DUMP>
+__device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink &hs, const DumpSink &dsk, int a0,
+                                               int a1, int b0, int b1, uint8_t *lds)
+{
+    constexpr int TB = 64, MB = 2, APW = 8;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int m = lane & 31, hf = lane >> 5;
+    v16i acc[MB][4];
+    fast_kloop<TA>(A, a0, b0, lds, acc);
 
     // Per-SNP class marginals of the tile, staged in LDS (the operand buffers
     // are free once every wave has left the K loop).
@@ -626,6 +656,382 @@ __global__ __launch_bounds__(fast_threads<TA>(), 64 / TA) void mfma_fast_kernel(
     HitSink hs = A.hs;
     hs.tag = A.S.tag;
     mfma_fast_tile<TA, DUMP>(A.S, hs, A.ds, T.x, T.y, T.z, T.w, lds);
+}
+
+// ------------------------------------------------------------- SPARSE ----
+// Tiles with missing calls in which every SNP misses few calls. The K loop is
+// the FAST tile's, over classes {0, 1} x {0, 1}; the NM row and column of X
+// then come from per-SNP marginals (M_x = sums over all calls of class x)
+// less sums over the few individuals that miss a call in the other SNP of
+// the pair (W = V, q):
+//   X[g][NM](a, b) = M_g(a) - sum_{i in miss(b)} I_g(a, i) W(i)    g = 0, 1, NM
+//   X[NM][h](a, b) = M_h(b) - sum_{i in miss(a)} I_h(b, i) W(i)    h = 0, 1
+//   sum of z over complete cases = Z_NM(a) - sum_{i in miss(b)} I_NM(a, i) z(i)
+// The sums run on the VALU in two passes with wave-uniform loops: the lanes
+// are 64 SNPs of one side of the tile and the loop runs over the miss list of
+// one SNP of the other side. The transposed bitplanes give the class bits of
+// those 64 SNPs at the individual as one 64-bit word, and each bit is its
+// lane's EXEC bit for the adds. Counts are exact. The FP32 sums feed only the
+// screen; a pair that passes it recomputes its sums exactly (INT64, one lane
+// per pair, sparse_exact) before the FP64 decision.
+//   Pass 1 (lanes = the 64 b-SNPs): wave w, for each of its eight a-SNPs,
+//   over miss(a): three values per pair into LDS, [wave][a][value][b].
+//   Pass 2 (lanes = the TA a-SNPs, lanes >= TA idle): wave w, for b-SNPs w,
+//   w + NW, ..., over miss(b): six values per pair into LDS, [value][a][b]
+//   with a row pitch of 65 dwords (a pass-2 store puts its lanes 65 dwords
+//   apart, which spreads them over all banks).
+constexpr int SP_PITCH = 65;    // dwords per row of pass-2 values
+
+// LDS of a SPARSE tile after its K loop: marginals of the TA + 64 SNPs
+// (n[3] int, q[3] and z double, q[3] and z float: 60 bytes each), pass-2
+// values, pass-1 values.
+template <int TA>
+constexpr int sp_mg_bytes() { return 60 * (TA + 64); }
+template <int TA>
+constexpr int sp_r2_bytes() { return 6 * TA * SP_PITCH * 4; }
+template <int TA>
+constexpr int sp_r1_bytes() { return (TA / 8) * 8 * 3 * 64 * 4; }
+
+template <int TA>
+constexpr int sparse_lds()
+{
+    return 2 * (2 * (TA + 64) + 5) * ROWB > sp_mg_bytes<TA>() + sp_r2_bytes<TA>() + sp_r1_bytes<TA>()
+               ? 2 * (2 * (TA + 64) + 5) * ROWB
+               : sp_mg_bytes<TA>() + sp_r2_bytes<TA>() + sp_r1_bytes<TA>();
+}
+
+// One individual of a pass-2 loop: the lanes whose bit is set in m0, m1, mn
+// (class 0, class 1, call present at their SNP) add v, qf (and zf for mn).
+// The wave-uniform masks are the lanes' conditions (selects with a scalar
+// mask operand), so the adds need no EXEC changes and do not wait on each
+// other.
+__device__ __forceinline__ void sp_add3(uint64_t m0, uint64_t m1, uint64_t mn, int v, float qf, float zf,
+                                        uint32_t &c0, float &s0, uint32_t &c1, float &s1, uint32_t &cn, float &sn,
+                                        float &zn)
+{
+    c0 += __builtin_amdgcn_inverse_ballot_w64(v ? m0 : 0ull);
+    c1 += __builtin_amdgcn_inverse_ballot_w64(v ? m1 : 0ull);
+    cn += __builtin_amdgcn_inverse_ballot_w64(v ? mn : 0ull);
+    s0 += __builtin_amdgcn_inverse_ballot_w64(m0) ? qf : 0.0f;
+    s1 += __builtin_amdgcn_inverse_ballot_w64(m1) ? qf : 0.0f;
+    sn += __builtin_amdgcn_inverse_ballot_w64(mn) ? qf : 0.0f;
+    zn += __builtin_amdgcn_inverse_ballot_w64(mn) ? zf : 0.0f;
+}
+
+// One individual of a pass-1 loop (classes 0 and 1 only).
+__device__ __forceinline__ void sp_add2(uint64_t m0, uint64_t m1, int v, float qf, uint32_t &c0, float &s0,
+                                        uint32_t &c1, float &s1)
+{
+    c0 += __builtin_amdgcn_inverse_ballot_w64(v ? m0 : 0ull);
+    c1 += __builtin_amdgcn_inverse_ballot_w64(v ? m1 : 0ull);
+    s0 += __builtin_amdgcn_inverse_ballot_w64(m0) ? qf : 0.0f;
+    s1 += __builtin_amdgcn_inverse_ballot_w64(m1) ? qf : 0.0f;
+}
+
+// The individuals of a pass are consumed one at a time by every lane (the
+// lanes are SNPs), but loaded 64 at a time, one per lane: a chunk is up to 64
+// consecutive entries of one miss list with, per lane, its individual's words
+// in the tile's 64-SNP column tb (32 bytes per individual: class 0, class 1,
+// call present) and its MissPhen head {v, qf, zf}. v_readlane then hands each
+// entry's words and values to the scalar unit for the EXEC-masked adds. A
+// chunk's loads are issued one chunk ahead, so their latency hides behind the
+// previous chunk's adds.
+struct SpChunk {
+    uint64_t m0, m1, mn;
+    int v;
+    float qf, zf;
+    int n;  // entries in the chunk (wave-uniform)
+};
+
+template <int NPL>
+__device__ __forceinline__ SpChunk sp_fetch(const MfmaScan &S, long long c, long long e1, const uint8_t *tb)
+{
+    typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+    typedef int v3 __attribute__((ext_vector_type(3)));
+    const int lane = threadIdx.x & 63;
+    SpChunk k;
+    k.n = (int)(e1 - c < 64 ? e1 - c : 64);
+    const uint32_t i = (uint32_t)S.mlist[c + (lane < k.n ? lane : 0)];
+    const uint32_t o = i << 5;
+    // exactly the words used (v, qf, zf), so that no later write to an unused
+    // register of the load has to wait for it
+    const u64x2 t = *reinterpret_cast<const u64x2 *>(tb + o);
+    const v3 w = *reinterpret_cast<const v3 *>(reinterpret_cast<const uint8_t *>(S.mp) + o);
+    k.m0 = t[0];
+    k.m1 = t[1];
+    k.mn = NPL == 3 ? *reinterpret_cast<const uint64_t *>(tb + o + 16) : 0ull;
+    k.v = w[0];
+    k.qf = __int_as_float(w[1]);
+    k.zf = __int_as_float(w[2]);
+    return k;
+}
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t x, int u)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, u);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), u);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Wave-uniform loop over the entries of nlist miss lists, list t being
+// mlist[moff[snp(t)], moff[snp(t) + 1]) (snp(t) < 0: an empty list);
+// begin(t) before a list's entries, fn(m0, m1, mn, v, qf, zf) per entry and
+// end(t) after them.
+template <int NPL, class Snp, class Begin, class Fn, class End>
+__device__ __forceinline__ void sp_lists(const MfmaScan &S, int nlist, const uint8_t *tb, Snp snp, Begin begin, Fn fn,
+                                         End end)
+{
+    int t = 0;
+    long long c = 0, e1 = 0;
+    auto bounds = [&](int tt, long long &cc, long long &ee) {
+        const int s = tt < nlist ? snp(tt) : -1;
+        cc = s >= 0 ? S.moff[s] : 0;
+        ee = s >= 0 ? S.moff[s + 1] : 0;
+    };
+    // first non-empty chunk
+    bounds(0, c, e1);
+    SpChunk cur = {};
+    if (c < e1) cur = sp_fetch<NPL>(S, c, e1, tb);
+    begin(0);
+    for (;;) {
+        // This chunk's words are in registers before the next chunk's loads go
+        // out (the load counter is in order, but a wait placed in the loop
+        // below would also wait for the next chunk).
+        asm volatile("" ::"v"(cur.m0), "v"(cur.m1), "v"(cur.mn), "v"(cur.v), "v"(cur.qf), "v"(cur.zf) : "memory");
+        // the next chunk: later in this list, or the first of a following non-empty list
+        int tn = t;
+        long long cn = c + 64, en = e1;
+        int skipped = 0;
+        while (cn >= en && tn < nlist) {
+            tn++;
+            skipped++;
+            bounds(tn, cn, en);
+        }
+        SpChunk nxt = {};
+        if (tn < nlist && cn < en) nxt = sp_fetch<NPL>(S, cn, en, tb);
+        if (c < e1)
+            for (int u = 0; u < cur.n; u++)
+                fn(readlane64(cur.m0, u), readlane64(cur.m1, u), NPL == 3 ? readlane64(cur.mn, u) : 0ull,
+                   __builtin_amdgcn_readlane(cur.v, u), __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.qf), u)),
+                   NPL == 3 ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.zf), u)) : 0.0f);
+        // lists finished before the next chunk
+        for (int k = 0; k < skipped; k++) {
+            end(t + k);
+            if (t + k + 1 < nlist) begin(t + k + 1);
+        }
+        if (tn >= nlist) break;
+        t = tn;
+        c = cn;
+        e1 = en;
+        cur = nxt;
+    }
+}
+
+// Exact (INT64) sums of pair (a, b) over the individuals missing a call in
+// one SNP, by the other SNP's class: over miss(b), a's classes 0, 1, NM into
+// dq[0..2] (and z over NM into dz); over miss(a), b's classes 0, 1 into eq.
+// One lane per pair; the loops are per lane.
+__device__ __forceinline__ void sparse_exact(const MfmaScan &S, int a, int b, long long dq[3], long long &dz,
+                                             long long eq[2])
+{
+    const int an = a < S.nsnp ? a : S.nsnp - 1, bn = b < S.nsnp ? b : S.nsnp - 1;
+    const size_t ps = (size_t)(S.Kp / 32) * S.nsnp;  // bitplane stride
+    dq[0] = dq[1] = dq[2] = dz = eq[0] = eq[1] = 0;
+    for (long long e = S.moff[bn]; e < S.moff[bn + 1]; e++) {
+        const int i = S.mlist[e];
+        const uint32_t bit = 1u << (i & 31);
+        const size_t o = (size_t)(i >> 5) * S.nsnp + an;
+        const long long q = S.mp[i].q;
+        if (S.bits[o] & bit) dq[0] += q;
+        if (S.bits[ps + o] & bit) dq[1] += q;
+        if (S.bits[2 * ps + o] & bit) {
+            dq[2] += q;
+            dz += S.mp[i].z;
+        }
+    }
+    for (long long e = S.moff[an]; e < S.moff[an + 1]; e++) {
+        const int i = S.mlist[e];
+        const uint32_t bit = 1u << (i & 31);
+        const size_t o = (size_t)(i >> 5) * S.nsnp + bn;
+        const long long q = S.mp[i].q;
+        if (S.bits[o] & bit) eq[0] += q;
+        if (S.bits[ps + o] & bit) eq[1] += q;
+    }
+}
+
+template <int TA, bool DUMP>
+__device__ __forceinline__ void mfma_sparse_tile(const MfmaScan &A, const HitSink &hs, const DumpSink &dsk, int a0,
+                                                 int a1, int b0, int b1, uint8_t *lds)
+{
+    constexpr int TB = 64, MB = 2, APW = 8, NW = TA / 8;
+    static_assert(sparse_lds<TA>() * (64 / TA) <= 160 * 1024, "LDS size");
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int m = lane & 31, hf = lane >> 5;
+    v16i acc[MB][4];
+    fast_kloop<TA>(A, a0, b0, lds, acc);
+    __syncthreads();  // the stage buffers are free
+
+    // Marginals of the tile's SNPs (a-SNPs first), pass-2 and pass-1 values.
+    constexpr int NS = TA + TB;
+    int *mg_n = reinterpret_cast<int *>(lds);                              // [3][NS]
+    double *mg_q = reinterpret_cast<double *>(lds + 12 * NS);              // [3][NS]
+    double *mg_z = reinterpret_cast<double *>(lds + 36 * NS);              // [NS]
+    float *mg_f = reinterpret_cast<float *>(lds + 44 * NS);                // [3][NS]
+    float *mg_zf = reinterpret_cast<float *>(lds + 56 * NS);               // [NS]
+    uint32_t *r2u = reinterpret_cast<uint32_t *>(lds + sp_mg_bytes<TA>()); // [6][TA][SP_PITCH]
+    float *r2f = reinterpret_cast<float *>(r2u);
+    uint32_t *r1u = reinterpret_cast<uint32_t *>(lds + sp_mg_bytes<TA>() + sp_r2_bytes<TA>());  // [NW][8][3][64]
+    float *r1f = reinterpret_cast<float *>(r1u);
+    if (tid < NS) {
+        int sn = tid < TA ? a0 + tid : b0 + tid - TA;
+        sn = sn < A.nsnp ? sn : A.nsnp - 1;
+        const Marg &mm = A.marg[sn];
+#pragma unroll
+        for (int g = 0; g < 3; g++) {
+            mg_n[NS * g + tid] = mm.n[g];
+            mg_q[NS * g + tid] = (double)mm.q[g];
+            mg_f[NS * g + tid] = (float)mm.q[g];
+        }
+        mg_z[tid] = (double)mm.zs;
+        mg_zf[tid] = (float)mm.zs;
+    }
+    // Pass 1: lanes = b-SNPs b0 + lane, over miss(a) for a = a0 + 8 wid + j.
+    {
+        const int vb = __builtin_amdgcn_readfirstlane(A.vidx[b0]);
+        const uint8_t *tb = reinterpret_cast<const uint8_t *>(A.tbits + (size_t)(vb >> 6) * A.tstride);
+        uint32_t c0 = 0, c1 = 0;
+        float s0 = 0.0f, s1 = 0.0f;
+        uint32_t *r1w = r1u + wid * (APW * 3 * 64);
+        sp_lists<2>(
+            A, APW, tb, [&](int j) { return a0 + APW * wid + j < a1 ? a0 + APW * wid + j : -1; },
+            [&](int) {
+                c0 = c1 = 0;
+                s0 = s1 = 0.0f;
+            },
+            [&](uint64_t m0, uint64_t m1, uint64_t, int v, float qf, float) { sp_add2(m0, m1, v, qf, c0, s0, c1, s1); },
+            [&](int j) {
+                r1w[(3 * j + 0) * 64 + lane] = c0 | (c1 << 16);
+                reinterpret_cast<float *>(r1w)[(3 * j + 1) * 64 + lane] = s0;
+                reinterpret_cast<float *>(r1w)[(3 * j + 2) * 64 + lane] = s1;
+            });
+    }
+    // Pass 2: lanes = a-SNPs a0 + lane, over miss(b) for b = b0 + wid + NW k.
+    {
+        const int va = __builtin_amdgcn_readfirstlane(A.vidx[a0]);
+        const uint8_t *ta = reinterpret_cast<const uint8_t *>(A.tbits + (size_t)(va >> 6) * A.tstride);
+        const int ash = va & 63;  // 0, or 32 for the second half of a 64-SNP word (TA = 32)
+        uint32_t c0 = 0, c1 = 0, cn = 0;
+        float s0 = 0.0f, s1 = 0.0f, sn = 0.0f, zn = 0.0f;
+        sp_lists<3>(
+            A, TB / NW, ta, [&](int k) { return b0 + wid + NW * k < b1 ? b0 + wid + NW * k : -1; },
+            [&](int) {
+                c0 = c1 = cn = 0;
+                s0 = s1 = sn = zn = 0.0f;
+            },
+            [&](uint64_t m0, uint64_t m1, uint64_t mn, int v, float qf, float zf) {
+                if (TA == 64)
+                    sp_add3(m0, m1, mn, v, qf, zf, c0, s0, c1, s1, cn, sn, zn);
+                else
+                    sp_add3(m0 >> ash, m1 >> ash, mn >> ash, v, qf, zf, c0, s0, c1, s1, cn, sn, zn);
+            },
+            [&](int k) {
+                if (lane < TA) {
+                    const int o = lane * SP_PITCH + wid + NW * k;
+                    r2u[0 * TA * SP_PITCH + o] = c0 | (c1 << 16);
+                    r2u[1 * TA * SP_PITCH + o] = cn;
+                    r2f[2 * TA * SP_PITCH + o] = s0;
+                    r2f[3 * TA * SP_PITCH + o] = s1;
+                    r2f[4 * TA * SP_PITCH + o] = sn;
+                    r2f[5 * TA * SP_PITCH + o] = zn;
+                }
+            });
+    }
+    __syncthreads();  // marginals and pass values are in LDS
+
+    const float zu = (float)A.zunit;
+#pragma unroll
+    for (int i = 0; i < MB; i++) {
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            const int al = APW * wid + 4 * i + 2 * hf + p;
+            const int a = a0 + al;
+#pragma unroll
+            for (int bg = 0; bg < 2; bg++) {
+                const int bl = 32 * bg + m, tb = TA + bl;
+                const int b = b0 + bl;
+                const int o = al * SP_PITCH + bl;
+                const uint32_t rc = r2u[o], rn = r2u[TA * SP_PITCH + o];
+                const float rs0 = r2f[2 * TA * SP_PITCH + o], rs1 = r2f[3 * TA * SP_PITCH + o];
+                const float rsn = r2f[4 * TA * SP_PITCH + o], rz = r2f[5 * TA * SP_PITCH + o];
+                const int jw = 4 * i + 2 * hf + p;  // a-SNP of the pair among the wave's eight
+                const uint32_t *r1p = r1u + (wid * APW + jw) * 3 * 64 + bl;
+                const uint32_t dc = r1p[0];
+                const float d1s0 = __uint_as_float(r1p[64]), d1s1 = __uint_as_float(r1p[128]);
+                int xc[3][3];
+                float xf[3][3];
+#pragma unroll
+                for (int g = 0; g < 2; g++)
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        const v16i &v = acc[i][2 * bg + h];
+                        const int e = 8 * p + 4 * g;
+                        xc[g][h] = v[e];
+                        xf[g][h] = (float)v[e + 1] + 256.0f * (float)v[e + 2] + 65536.0f * (float)v[e + 3];
+                    }
+                xc[0][2] = mg_n[al] - (int)(rc & 0xFFFFu);
+                xc[1][2] = mg_n[NS + al] - (int)(rc >> 16);
+                xc[2][2] = mg_n[2 * NS + al] - (int)rn;
+                xc[2][0] = mg_n[tb] - (int)(dc & 0xFFFFu);
+                xc[2][1] = mg_n[NS + tb] - (int)(dc >> 16);
+                xf[0][2] = mg_f[al] - rs0;
+                xf[1][2] = mg_f[NS + al] - rs1;
+                xf[2][2] = mg_f[2 * NS + al] - rsn;
+                xf[2][0] = mg_f[tb] - d1s0;
+                xf[2][1] = mg_f[NS + tb] - d1s1;
+                float sqf = (mg_zf[al] - rz) * zu;
+                if (A.nout) add_outliers(A, a, b, xf, sqf);
+                int n[9];
+                float sf[9];
+                cells_from_x(xc, n);
+                cells_from_x(xf, sf);
+                // integer-valued doubles below 2^53: exact
+                auto exact = [&](double sd[9], double &sq) {
+                    double xs[3][3];
+#pragma unroll
+                    for (int g = 0; g < 2; g++)
+#pragma unroll
+                        for (int h = 0; h < 2; h++) {
+                            const v16i &v = acc[i][2 * bg + h];
+                            const int e = 8 * p + 4 * g;
+                            xs[g][h] = (double)v[e + 1] + 256.0 * (double)v[e + 2] + 65536.0 * (double)v[e + 3];
+                        }
+                    long long dq[3], dz, eq[2];
+                    sparse_exact(A, a, b, dq, dz, eq);
+                    xs[0][2] = mg_q[al] - (double)dq[0];
+                    xs[1][2] = mg_q[NS + al] - (double)dq[1];
+                    xs[2][2] = mg_q[2 * NS + al] - (double)dq[2];
+                    xs[2][0] = mg_q[tb] - (double)eq[0];
+                    xs[2][1] = mg_q[NS + tb] - (double)eq[1];
+                    sq = (mg_z[al] - (double)dz) * A.zunit;
+                    if (A.nout) add_outliers(A, a, b, xs, sq);
+                    cells_from_x(xs, sd);
+                };
+                const bool valid = a < a1 && b < b1 && a > b;
+                finish_pair<DUMP>(hs, dsk, valid, a, b, n, sf, sqf, exact, CLS_IDENTITY, CLS_IDENTITY);
+            }
+        }
+    }
+}
+
+template <int TA, bool DUMP>
+__global__ __launch_bounds__(fast_threads<TA>(), 64 / TA) void mfma_sparse_kernel(const MfmaArgs A)
+{
+    __shared__ __attribute__((aligned(16))) uint8_t lds[sparse_lds<TA>()];
+    const int4 T = A.tiles[xcd_tile(blockIdx.x, A.ntiles)];
+    HitSink hs = A.hs;
+    hs.tag = A.S.tag;
+    mfma_sparse_tile<TA, DUMP>(A.S, hs, A.ds, T.x, T.y, T.z, T.w, lds);
 }
 
 // ------------------------------------------------------------- GENERAL ----

@@ -19,10 +19,10 @@ __device__ __forceinline__ int decode(const uint32_t *w, int npack, int remain, 
     return (int)((w[j] >> sh) & 3u);
 }
 
-// One thread per SNP: order the genotype classes by frequency and flag SNPs
-// with missing calls. The plain form stores the two rarest classes as
-// bitplanes 0 and 1 and derives the most common one; cls packs plane -> class
-// (2 bits each).
+// One thread per SNP: order the genotype classes by frequency and count the
+// missing calls. The plain form stores the two rarest classes as bitplanes 0
+// and 1 and derives the most common one; cls packs plane -> class (2 bits
+// each).
 __global__ void class_order(const uint32_t *words, int nid, int npack, int remain, int npack1, int nsnp,
                             int32_t *cls, int32_t *miss)
 {
@@ -37,7 +37,7 @@ __global__ void class_order(const uint32_t *words, int nid, int npack, int remai
     int p0 = big == 0 ? 1 : 0;
     int p1 = big == 2 ? 1 : 2;
     cls[s] = p0 | (p1 << 2) | (big << 4);
-    miss[s] = n[3] > 0;
+    miss[s] = n[3];
 }
 
 // One thread per (word of 32 individuals, snp): bitplanes [3][Kw][nsnp] of
@@ -70,15 +70,17 @@ __global__ void transcode_bits(const uint32_t *words, int nid, int npack, int re
 // groups of threads split the words and are reduced in LDS.
 __global__ __launch_bounds__(256) void marginals_kernel(const uint32_t *mbits, int Kw, int nsnp,
                                                         const uint32_t *vbits, const int32_t *qv,
-                                                        const float *yc, const int32_t *cls, Marg *marg)
+                                                        const float *yc, const int32_t *cls, const MissPhen *mp,
+                                                        Marg *marg)
 {
     __shared__ long long sq[4][3][64];
     __shared__ double sy[4][3][64];
     __shared__ int sn[4][3][64];
+    __shared__ long long sz[4][64];
     const int sl = threadIdx.x & 63, grp = threadIdx.x >> 6;
     const int s = blockIdx.x * 64 + sl;
     int n[3] = {0, 0, 0};
-    long long q[3] = {0, 0, 0};
+    long long q[3] = {0, 0, 0}, z = 0;
     double y[3] = {0.0, 0.0, 0.0};
     if (s < nsnp) {
         for (int wd = grp; wd < Kw; wd += 4) {
@@ -96,6 +98,7 @@ __global__ __launch_bounds__(256) void marginals_kernel(const uint32_t *mbits, i
                     x &= x - 1;
                     q[c] += qv[k];
                     y[c] += (double)yc[k];
+                    z += mp[k].z;
                 }
             }
         }
@@ -105,6 +108,7 @@ __global__ __launch_bounds__(256) void marginals_kernel(const uint32_t *mbits, i
         sq[grp][c][sl] = q[c];
         sy[grp][c][sl] = y[c];
     }
+    sz[grp][sl] = z;
     __syncthreads();
     if (grp == 0 && s < nsnp) {
         int tn[3];
@@ -133,6 +137,7 @@ __global__ __launch_bounds__(256) void marginals_kernel(const uint32_t *mbits, i
         mg.ys[2] = ty[0] + ty[1] + ty[2];
         mg.cls = cl;
         mg.pad = 0;
+        mg.zs = sz[0][sl] + sz[1][sl] + sz[2][sl] + sz[3][sl];
         marg[s] = mg;
     }
 }
@@ -158,6 +163,54 @@ __global__ __launch_bounds__(256) void outlier_codes_kernel(const uint32_t *mbit
             code |= g << (2 * t);
         }
         ocodes[(size_t)w * nsnp + s] = code;
+    }
+}
+
+// One thread per SNP: the individuals (below nid) without a call, ascending,
+// into mlist[moff[s] ...].
+__global__ __launch_bounds__(256) void miss_list_kernel(const uint32_t *mbits, int Kw, int nsnp, int nid,
+                                                        const long long *moff, int32_t *mlist)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsnp) return;
+    long long o = moff[s];
+    for (int wd = 0; wd < Kw && 32 * wd < nid; wd++) {
+        uint32_t x = ~mbits[((size_t)2 * Kw + wd) * nsnp + s];
+        if (nid - 32 * wd < 32) x &= (1u << (nid - 32 * wd)) - 1u;
+        while (x) {
+            mlist[o++] = 32 * wd + __ffs(x) - 1;
+            x &= x - 1u;
+        }
+    }
+}
+
+// Transposed bitplanes for the SPARSE tiles: one wave per (64-SNP word wv of
+// the virtual SNP order, 32-individual word wd). Lane l is virtual SNP
+// 64 wv + l (snp_of_v: its SNP, or -1 for padding); a ballot over the lanes
+// of bit k of a plane word is the 64-SNP word of individual 32 wd + k, which
+// lane k collects and stores: tbits[wv][32 wd + k][plane], planes class 0,
+// class 1, call present, and a zero word (a 64-SNP column is contiguous).
+__global__ __launch_bounds__(64) void tbits_kernel(const uint32_t *mbits, int Kw, int nsnp, int nid,
+                                                    const int32_t *snp_of_v, int nw, uint64_t *tbits)
+{
+    const int wv = blockIdx.x, wd = blockIdx.y, l = threadIdx.x;
+    const int s = snp_of_v[64 * wv + l];
+    uint32_t lo[3] = {0u, 0u, 0u}, hi[3] = {0u, 0u, 0u};
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        const uint32_t x = s >= 0 ? mbits[((size_t)p * Kw + wd) * nsnp + s] : 0u;
+        for (int k = 0; k < 32; k++) {
+            const uint64_t b = __ballot((x >> k) & 1u);
+            lo[p] = l == k ? (uint32_t)b : lo[p];
+            hi[p] = l == k ? (uint32_t)(b >> 32) : hi[p];
+        }
+    }
+    const int i = 32 * wd + l;
+    if (l < 32 && i < nid) {
+        uint64_t *o = tbits + ((size_t)wv * nid + i) * 4;
+#pragma unroll
+        for (int p = 0; p < 3; p++) o[p] = ((uint64_t)hi[p] << 32) | lo[p];
+        o[3] = 0ull;
     }
 }
 

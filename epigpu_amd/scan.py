@@ -52,17 +52,18 @@ class ScanParams(ctypes.Structure):
 
 class Schedule(ctypes.Structure):
     _fields_ = [("fast_rows", ctypes.c_int32), ("super_a", ctypes.c_int32), ("super_b", ctypes.c_int32),
-                ("queues", ctypes.c_int32), ("order", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+                ("queues", ctypes.c_int32), ("order", ctypes.c_int32), ("dense_missing", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 2)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k in ("fast_rows", "super_a", "super_b", "queues", "order")}
+        return {k: getattr(self, k) for k in ("fast_rows", "super_a", "super_b", "queues", "order", "dense_missing")}
 
 
 class TuneEntry(ctypes.Structure):
     _fields_ = [("sched", Schedule), ("ms", ctypes.c_double)]
 
 
-TUNE_ROWS, TUNE_SUPER, TUNE_QUEUES, TUNE_ORDER, TUNE_ALL = 1, 2, 4, 8, 15
+TUNE_ROWS, TUNE_SUPER, TUNE_QUEUES, TUNE_ORDER, TUNE_MISSING, TUNE_ALL = 1, 2, 4, 8, 16, 31
 
 
 class RunOpts(ctypes.Structure):
@@ -154,6 +155,7 @@ def _declare(name, L):
             "epi_get_hits": (ctypes.c_int, [vp, vp, vp, vp, vp, i64, _p(i64)]),
             "epi_set_hit_capacity": (ctypes.c_int, [vp, i64]),
             "epi_last_scan_stats": (ctypes.c_int, [vp, _p(f64), _p(i64), _p(i32)]),
+            "epi_last_tiles": (ctypes.c_int, [vp, _p(i64), _p(i64), _p(i64)]),
             "epi_pair_dump": (ctypes.c_int, [vp, _p(ScanParams), i32, i32, i32, i32, vp, vp, vp, vp]),
             "epi_batch_create": (ctypes.c_int, [vp, _p(vp)]),
             "epi_batch_destroy": (None, [vp]),
@@ -663,6 +665,13 @@ class Scanner:
             g = got.value
             out.update(hits=hits[:g], F64=F[:g], F2_64=F2[:g], block=blk[:g])
         return out
+
+    def last_tiles(self):
+        """Tiles of the last scan, no-allocation scan or dump by kind."""
+        f, sp, g = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        self._chk(self.L.epi_last_tiles(self.ctx, ctypes.byref(f), ctypes.byref(sp), ctypes.byref(g)),
+                  "epi_last_tiles")
+        return {"fast": f.value, "sparse": sp.value, "general": g.value}
 
     def scan_noalloc(self, hits_dev, cap, test="f", form="mfma", t1=6.5, t2=10.5, phen=0, blk_begin=0, blk_end=0,
                      safe=False):

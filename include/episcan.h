@@ -188,16 +188,25 @@ int epi_set_max_tiles(epi_ctx *ctx, int64_t max_tiles);
 /* Tile shape and work order of a scan. The reference's only knob here is
  * the chunk size of its launches (-i, src/EpiOpenCL.c:1559-1607); these
  * settings replace it. A FAST tile (no missing call in either SNP range) is
- * fast_rows a-SNPs x 64 b-SNPs; a GENERAL tile is fast_rows x 32 in the MFMA
- * form and 16 x 64 in the plain form. */
+ * fast_rows a-SNPs x 64 b-SNPs. In the MFMA form a tile with missing calls is
+ * either a GENERAL tile, fast_rows x 32 (the full {0, 1, NM} contraction), or
+ * a SPARSE tile of the FAST shape (the FAST contraction plus per-pair sums
+ * over the individuals with a missing call, which pay off at low missing
+ * rates). A tile can be SPARSE when every SNP of its ranges misses at most
+ * max(8, nid / 16) calls; dense_missing 0 makes it SPARSE when its SNPs
+ * also miss at most 0.5 % of their calls on average. The plain form's GENERAL
+ * tiles are 16 x 64. */
 typedef struct epi_schedule {
     int32_t fast_rows;  /* MFMA form: 64 or 32; plain form: 32 or 16 */
     int32_t super_a;    /* tiles are issued in super-tiles of super_a x super_b */
     int32_t super_b;    /*   tiles (1..64 each) for L2 reuse */
     int32_t queues;     /* batch only: 8 = one work queue per XCD, 1 = one shared queue */
-    int32_t order;      /* batch only: 0 locality, 1 GENERAL tiles first,
+    int32_t order;      /* batch only: 0 locality, 1 GENERAL and SPARSE tiles first,
                            2 datasets by decreasing nid */
-    int32_t reserved[3];
+    int32_t dense_missing; /* MFMA form, tiles with missing calls: 0 = SPARSE at
+                              low missing rates (default), 1 = always GENERAL,
+                              2 = SPARSE wherever allowed */
+    int32_t reserved[2];
 } epi_schedule;
 
 /* Schedule of single scans of one form (default: fast_rows 32, super-tiles
@@ -210,7 +219,8 @@ int epi_get_schedule(epi_ctx *ctx, int32_t form, epi_schedule *s);
 #define EPI_TUNE_SUPER 2   /* super_a x super_b in {8x4, 4x4, 16x2, 4x8, 1x1} */
 #define EPI_TUNE_QUEUES 4  /* queues in {8, 1} */
 #define EPI_TUNE_ORDER 8   /* order in {0, 1, 2} */
-#define EPI_TUNE_ALL 15
+#define EPI_TUNE_MISSING 16 /* dense_missing in {0, 1, 2} */
+#define EPI_TUNE_ALL 31
 
 typedef struct epi_tune_entry {
     epi_schedule sched;
@@ -232,6 +242,11 @@ int epi_autotune(epi_ctx *ctx, const epi_scan_params *p, int32_t dims, int32_t r
 /* Device time (ms, HIP events on the context's stream) of the pair kernels
  * of the last epi_scan, and the number of pairs it tested. */
 int epi_last_scan_stats(epi_ctx *ctx, double *kernel_ms, int64_t *pairs, int32_t *launches);
+
+/* Tiles of the last epi_scan, epi_scan_noalloc or epi_pair_dump by kind:
+ * FAST, SPARSE and GENERAL (see epi_schedule). No counterpart in the
+ * reference; for tests and measurement. */
+int epi_last_tiles(epi_ctx *ctx, int64_t *fast, int64_t *sparse, int64_t *general);
 
 /* Debug / parity entry: every pair (a, b) with a0 <= a < a1, b0 <= b < b1 and
  * a > b, through the same kernels as epi_scan, writes its 3x3 table and FP64

@@ -146,7 +146,9 @@ def test_batch_overflow_and_errors(scanner, batch_data):
 
 
 SCHEDULES = {
-    "mfma": [dict(fast_rows=64), dict(fast_rows=64, super_a=1, super_b=1), dict(fast_rows=32, super_a=16, super_b=2)],
+    "mfma": [dict(fast_rows=64), dict(fast_rows=64, super_a=1, super_b=1), dict(fast_rows=32, super_a=16, super_b=2),
+             dict(fast_rows=64, dense_missing=1), dict(fast_rows=32, dense_missing=1),
+             dict(fast_rows=64, dense_missing=2), dict(fast_rows=32, dense_missing=2)],
     "plain": [dict(fast_rows=16), dict(fast_rows=32, super_a=4, super_b=8)],
 }
 
@@ -190,7 +192,8 @@ def test_batch_autotune(scanner, batch_data):
         assert np.abs(g["F64"] - h["F64"]).max(initial=0) <= 1e-12 * max(1.0, np.abs(h["F64"]).max(initial=0))
     # every schedule the tuner can pick scans the same pairs
     for sched in res["log"]:
-        b.set_schedule("mfma", **{k: sched[k] for k in ("fast_rows", "super_a", "super_b", "queues", "order")})
+        b.set_schedule("mfma", **{k: sched[k] for k in ("fast_rows", "super_a", "super_b", "queues", "order",
+                                                        "dense_missing")})
         st = b.scan(test="i", form="mfma", t1=T1, t2=T2)
         assert st["nhits"] == base["nhits"] and st["pairs"] == base["pairs"]
     b.close()

@@ -162,12 +162,13 @@ def test_single_scan_beyond_one_launch(scanner):
         scanner.set_schedule("mfma", **default)
 
 
-@pytest.mark.parametrize("form", ["plain", "mfma"])
-def test_scan_in_pieces_matches_one_list(scanner, form):
+@pytest.mark.parametrize("form,dense", [("plain", 0), ("mfma", 0), ("mfma", 1)])
+def test_scan_in_pieces_matches_one_list(scanner, form, dense):
     """A tile list above the context's bound is built, uploaded and launched in pieces (whole rows of
     super-tiles, across chromosome blocks). With the bound cut to 7, 60 and 500 tiles the scan must find exactly
     the hits and pair count of the one-list scan, for the whole range and for a block sub-range; afterwards the
-    default bound gives the one-list scan again. Some SNPs have missing calls, so FAST and GENERAL tiles mix."""
+    default bound gives the one-list scan again. Some SNPs have missing calls, so FAST tiles mix with SPARSE
+    tiles (MFMA form, dense_missing 0) or GENERAL tiles (dense_missing 1, and the plain form)."""
     import epigpu_amd as E
     nid, nsnp = 300, 1500
     words, y = make_dataset(0x91E, nid, nsnp, maf=(0.05, 0.5), gmiss=0.0)
@@ -179,17 +180,24 @@ def test_scan_in_pieces_matches_one_list(scanner, form):
     scanner.set_phenotypes(y)
     scanner.set_chromosomes(starts, sizes)
     scanner.set_hit_capacity(1 << 22)
+    default = scanner.schedule("mfma")
+    scanner.set_schedule("mfma", dense_missing=dense)
     key = lambda r: (r["hits"]["snp1"], r["hits"]["snp2"], r["block"])
     try:
         for bb, be in [(0, 0), (2, 8)]:
             one = scanner.scan(test="i", form=form, t1=2.0, t2=2.5, blk_begin=bb, blk_end=be)
             assert one["nhits"] > 1000 and not one["overflow"]
+            tiles = scanner.last_tiles()
+            if form == "mfma":
+                assert tiles["sparse" if dense == 0 else "general"] > 0, tiles
+            ntiles = sum(tiles.values())
             for cap in (7, 60, 500):
-                print(f"pieces form={form} blocks=[{bb},{be}) max_tiles={cap}", flush=True)
+                print(f"pieces form={form} dense={dense} blocks=[{bb},{be}) max_tiles={cap} of {ntiles}", flush=True)
                 scanner.set_max_tiles(cap)
                 r = scanner.scan(test="i", form=form, t1=2.0, t2=2.5, blk_begin=bb, blk_end=be)
                 assert r["pairs"] == one["pairs"] and r["nhits"] == one["nhits"] and not r["overflow"]
-                assert r["launches"] > one["launches"]
+                if ntiles > cap:
+                    assert r["launches"] > one["launches"]
                 for x, z in zip(key(r), key(one)):
                     assert (x == z).all()
                 assert np.array_equal(r["F64"], one["F64"]) and np.array_equal(r["F2_64"], one["F2_64"])
@@ -198,6 +206,7 @@ def test_scan_in_pieces_matches_one_list(scanner, form):
             assert again["launches"] == one["launches"] and again["nhits"] == one["nhits"]
     finally:
         scanner.set_max_tiles(1 << 25)
+        scanner.set_schedule("mfma", **default)
 
 
 @pytest.mark.parametrize("gmiss", [0.0, 0.001])
