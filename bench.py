@@ -338,6 +338,12 @@ def main():
     ap.add_argument("--schedule", default="", metavar="ROWS,SA,SB[,QUEUES,ORDER]",
                     help="fix the schedule (FAST tile height, super-tile; batch: queues, order) instead of autotuning")
     ap.add_argument("--c5-snps", type=int, default=0, help="c5: scan only the first K SNPs (0: all 200000)")
+    ap.add_argument("--gmiss", type=float, default=-1.0,
+                    help="single-scan configs: genotype missing-call rate instead of the workload's own (changes the "
+                         "workload, which the JSON's config then names)")
+    ap.add_argument("--dense-missing", type=int, default=-1, choices=[-1, 0, 1, 2],
+                    help="MFMA tiles with missing calls: 0 SPARSE at low rates, 1 GENERAL, 2 SPARSE wherever allowed "
+                         "(-1: the schedule's or the autotuner's choice)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run an elementwise copy of MIB MiB (FP32 y = x + 1) for HBM counter calibration")
     args = ap.parse_args()
@@ -382,6 +388,9 @@ def main():
         return
 
     wl = workloads.get(args.config)
+    if args.gmiss >= 0:
+        wl.gmiss = args.gmiss
+        wl.name = f"{wl.name} with {args.gmiss:g} missing calls"
     t0 = time.perf_counter()
     wl.generate()
     log(f"[bench] generated {wl.name}: N={wl.nid} M={wl.nsnp} pairs={wl.pairs} in {time.perf_counter() - t0:.1f} s")
@@ -393,6 +402,8 @@ def main():
     if args.schedule:
         rows, sa, sb = (int(x) for x in args.schedule.split(",")[:3])
         s.set_schedule(args.form, fast_rows=rows, super_a=sa, super_b=sb)
+    if args.dense_missing >= 0:
+        s.set_schedule(args.form, dense_missing=args.dense_missing)
     t0 = time.perf_counter()
     s.set_genotypes(wl.words, wl.nid)
     s.set_phenotypes(wl.y)
@@ -404,7 +415,7 @@ def main():
         return s.scan(test=wl.test, form=form, t1=wl.t1, t2=wl.t2)
 
     tuned = None
-    if not args.no_tune and not args.fast_rows and not args.schedule:
+    if not args.no_tune and not args.fast_rows and not args.schedule and args.dense_missing < 0:
         t0 = time.perf_counter()
         tuned = s.autotune(test=wl.test, form=args.form, t1=wl.t1, t2=wl.t2, reps=1)
         tuned["tune_s"] = time.perf_counter() - t0
@@ -429,6 +440,7 @@ def main():
         found = sum((int(a), int(b)) in hs for a, b, _, _ in wl.planted_pairs)
 
     kernel_ms = float(np.mean(kms))
+    tiles = s.last_tiles()
     traffic = profiled_traffic(wl.name, args.form, s.schedule(args.form))
     out = {
         "metric": METRIC, "value": value, "unit": "pair-tests/s", "n_gpus": 1, "steps": args.steps,
@@ -443,7 +455,7 @@ def main():
         "roofline": roofline(args.form, wl, kernel_ms, traffic,
                              PROFILED.get((wl.name, args.form, s.schedule(args.form)["fast_rows"]), (None,))[0]),
         "hits": int(r["nhits"]), "planted_found": found,
-        "prep_ms": prep_s * 1e3,
+        "prep_ms": prep_s * 1e3, "tiles": tiles,
     }
     if not args.no_plain and args.form == "mfma":
         r2 = step("plain")

@@ -276,7 +276,7 @@ int sparse_max(int nid) { return std::min(65535, std::max(8, nid / 16)); }
 // N = 4000 with 64-row tiles, the per-individual sums of a SPARSE tile cost
 // less than the NM rows of a GENERAL tile up to about 0.7 % missing calls
 // (0.5 %: 17.7 ms against 20.5 ms for the scan; 1 %: 24.7 against 20.5).
-constexpr long long SPARSE_RATE = 200;
+constexpr long long SPARSE_RATE = 150;
 
 int ds_upload_genotypes(epi_ctx *c, Dataset &d, const int32_t *words, int nid, int nsnp)
 {

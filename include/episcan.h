@@ -194,7 +194,7 @@ int epi_set_max_tiles(epi_ctx *ctx, int64_t max_tiles);
  * over the individuals with a missing call, which pay off at low missing
  * rates). A tile can be SPARSE when every SNP of its ranges misses at most
  * max(8, nid / 16) calls; dense_missing 0 makes it SPARSE when its SNPs
- * also miss at most 0.5 % of their calls on average. The plain form's GENERAL
+ * also miss at most 1/150 of their calls on average. The plain form's GENERAL
  * tiles are 16 x 64. */
 typedef struct epi_schedule {
     int32_t fast_rows;  /* MFMA form: 64 or 32; plain form: 32 or 16 */

@@ -2,7 +2,7 @@
 the FAST contraction plus per-pair sums over the individuals that miss a call.
 
 Most tests force SPARSE tiles wherever they are allowed (dense_missing = 2), so that missing rates above the
-default's 0.5 % run through them too. Checks: per-pair tables bit-exact and F / F2 within 1e-5 of the oracle, and
+default's 0.67 % run through them too. Checks: per-pair tables bit-exact and F / F2 within 1e-5 of the oracle, and
 cell sums and F identical to the GENERAL tiles' (dense_missing = 1) on the same data, since both end in the same
 exact integer sums; one dataset whose scan has FAST, SPARSE and GENERAL tiles; the default's choice by missing
 rate; outliers and safe mode; whole scans' hit lists against the oracle on chromosomes whose sizes are not
