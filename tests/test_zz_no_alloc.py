@@ -1,7 +1,8 @@
 """epi_scan_noalloc, the scan entry point that allocates no device memory, through the C ABI: its hits
 (unsorted, FP64 records in a device buffer the caller allocated beforehand) equal those of epi_scan on the same
-context, record for record, on two calls in a row of which the second scans a smaller dataset (the
-library's fixed scratch keeps the previous call's tile list and counter); a hit buffer smaller than the hit
+context, record for record, on three calls in a row, each on a smaller dataset than the one before (the
+library's fixed scratch keeps the previous call's tile list and counter), with GENERAL, SPARSE and FAST tiles
+in turn; a hit buffer smaller than the hit
 count receives a subset with the full count reported; a scan beyond the scratch's tile capacity is refused
 before any device work."""
 import ctypes
@@ -45,7 +46,8 @@ def test_noalloc_equals_scan_on_two_calls(scanner, form):
     cap = 1 << 16
     buf = DevBuf(cap * S.HIT64_DTYPE.itemsize)
     try:
-        cases = [(0x40A, 900, [300, 260], 0.01), (0x40B, 300, [150], 0.0)]  # the second call: a smaller dataset
+        # GENERAL tiles (1 % missing calls), SPARSE tiles (0.3 %), then a smaller dataset (FAST tiles)
+        cases = [(0x40A, 900, [300, 260], 0.01), (0x40D, 700, [200, 180], 0.003), (0x40B, 300, [150], 0.0)]
         for seed, nid, sizes, gmiss in cases:
             nsnp = sum(sizes)
             words, y = make_dataset(seed, nid, nsnp, gmiss=gmiss, pmiss=0.02)
@@ -54,7 +56,11 @@ def test_noalloc_equals_scan_on_two_calls(scanner, form):
             scanner.set_chromosomes(list(np.cumsum([0] + sizes[:-1])), sizes)
             kw = dict(test="i", form=form, t1=2.8, t2=3.6)
             g = scanner.scan(**kw)
+            kinds = scanner.last_tiles()
             n = scanner.scan_noalloc(buf.ptr.value, cap, **kw)
+            assert scanner.last_tiles() == kinds
+            if form == "mfma" and gmiss == 0.003:
+                assert kinds["sparse"] > 0, kinds
             print(f"noalloc form={form} nid={nid} nsnp={nsnp}: {n} hits (epi_scan {g['nhits']})", flush=True)
             assert n == g["nhits"] > 50 and n <= cap
             h = sorted_records(buf.read(S.HIT64_DTYPE, n))
