@@ -187,3 +187,16 @@ def test_sparse_batch_equals_single_scans(sch, rows):
             assert sch.last_tiles()["sparse"] > 0
             assert set(zip(g["hits"]["snp1"].tolist(), g["hits"]["snp2"].tolist())) == got[sid]
             assert len(got[sid]) > 50
+
+
+@pytest.mark.parametrize("rows", [32, 64])
+def test_sparse_long_miss_lists(sch, rows):
+    """N = 20,001 at 0.5 % missing calls: about 100 individuals per SNP's miss list, so the passes load each
+    list in several chunks of 64 (and lists of every length around the chunk boundary occur)."""
+    nid, nsnp = 20001, 130
+    words, y = make_dataset(0x10C5 + rows, nid, nsnp, maf=(0.02, 0.5), gmiss=0.005, pmiss=0.02)
+    import epigpu_amd as E
+    m = (E.unpack(words, nid) == 3).sum(1)
+    assert m.max() > 64 and m.min() < 128
+    out = dump_kinds(sch, words, y, nid, nsnp, rows)
+    assert out[2][4]["sparse"] > 0
