@@ -180,6 +180,67 @@ __device__ __forceinline__ bool screen32(const int n[9], const float s[9], float
     return 0.25f * ssi * (float)d2 > t2s * ssw;
 }
 
+// Branch-free first stage of screen32: false only if screen32 is false for
+// certain, i.e. the pair fails the degrees-of-freedom and nfac rules, or it
+// clears neither the SSW guard, nor the F threshold, nor the F2 bound (F2 <=
+// d1 F / 4, see screen32). Straight-line code, so that a lane's pairs can be
+// screened side by side; the few pairs it keeps go through screen32.
+__device__ __forceinline__ bool screen_maybe(const int n[9], const float s[9], float sq, int test, int safe,
+                                             float t1s, float t2s)
+{
+    int N = 0, nfac = 0;
+    float tot = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+        N += n[c];
+        tot += s[c];
+        nfac += n[c] > 0;
+    }
+    const int d1 = nfac - 1, d2 = N - nfac;
+    const bool dfok = d1 >= 1 && d2 >= 1 && (test == 'f' || safe || nfac > 6);
+    const float rN = __builtin_amdgcn_rcpf((float)N);
+    const float mbar = tot * rN;
+    float ssb = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+        const float m = n[c] > 0 ? s[c] * __builtin_amdgcn_rcpf((float)n[c]) : 0.0f;
+        const float e = m - mbar;
+        ssb += (float)n[c] * e * e;
+    }
+    const float ssw = sq - (float)N * mbar * mbar - ssb;
+    const bool guard = !(ssw > 1e-4f * (sq + (float)N * mbar * mbar + ssb));
+    const bool f1 = ssb * (float)d2 > t1s * (float)d1 * ssw;
+    const bool f2 = test != 'f' && 0.25f * ssb * (float)d2 > t2s * ssw;
+    return dfok && (guard || f1 || f2);
+}
+
+// First stage of the screen for FAST tiles, where every present individual is
+// a complete case, so that N, the phenotype total T and SST = sq - T^2 / N are
+// the same for every pair: SSB = sum_c s_c^2 / n_c - T^2 / N needs one
+// reciprocal and two multiply-adds per cell. n are the counts as FP32 (exact
+// below 2^24). False only if screen32 is false for certain (the same rules and
+// the same 2% margin; FP32 rounding moves SSB by far less than the margin,
+// and an SSW below 1e-4 of the terms it is the difference of defers to FP64,
+// as in screen32).
+__device__ __forceinline__ bool screen_fast_maybe(const float n[9], const float s[9], float N, float T2N, float sq,
+                                                  int test, int safe, float t1s, float t2s)
+{
+    float nfac = 0.0f, ssb = -T2N;
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+        const bool occ = n[c] > 0.0f;
+        nfac += occ ? 1.0f : 0.0f;
+        ssb += occ ? s[c] * s[c] * __builtin_amdgcn_rcpf(n[c]) : 0.0f;
+    }
+    const float d1 = nfac - 1.0f, d2 = N - nfac;
+    const bool dfok = d1 >= 1.0f && d2 >= 1.0f && (test == 'f' || safe || nfac > 6.0f);
+    const float ssw = sq - T2N - ssb;
+    const bool guard = !(ssw > 1e-4f * (sq + T2N + ssb));
+    const bool f1 = ssb * d2 > t1s * d1 * ssw;
+    const bool f2 = test != 'f' && 0.25f * ssb * d2 > t2s * ssw;
+    return dfok && (guard || f1 || f2);
+}
+
 // Wave-aggregated append: one atomic per wave and call. Every lane of the
 // wave must call it (hit may be false).
 __device__ __forceinline__ void append_hit(const HitSink &hs, bool hit, const DevHit &h)

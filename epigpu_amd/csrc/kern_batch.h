@@ -88,7 +88,9 @@ __global__ __launch_bounds__(fast_threads<TA>(), 64 / TA) void mfma_batch_kernel
         const int4 w = wsh;
         const int a0 = __builtin_amdgcn_readfirstlane(w.x), b0 = __builtin_amdgcn_readfirstlane(w.y);
         const int ext = __builtin_amdgcn_readfirstlane(w.z), sc = __builtin_amdgcn_readfirstlane(w.w);
-        const MfmaScan S = B.mscan[sc];
+        // the scan's descriptor stays in memory (its fields load where used)
+        // rather than held in registers across the tile
+        const MfmaScan &S = B.mscan[sc];
         HitSink hs = B.hs;
         hs.tag = sc;
         const int a1 = a0 + (ext & 255), b1 = b0 + ((ext >> 8) & 255);

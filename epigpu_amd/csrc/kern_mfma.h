@@ -166,6 +166,9 @@ struct MfmaArgs {
 };
 
 constexpr int FAST_LDS = 2 * (2 * (64 + 64) + 5) * ROWB;     // largest FAST tile, two stage buffers
+constexpr int FAST_MG_LDS = 4096;                             // a FAST tile's marginals (128 SNPs x 32 bytes)
+template <int TA>
+constexpr int fast_stage_lds() { return 2 * (2 * (TA + 64) + 5) * ROWB; }
 
 // blockIdx -> tile: consecutive tiles (which share SNP data) go to the same
 // XCD. Workgroups are dealt to the 8 XCDs round-robin, so within each full
@@ -495,15 +498,13 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
     constexpr int TB = 64, MB = 2, APW = 8;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int m = lane & 31, hf = lane >> 5;
-    v16i acc[MB][4];
-    fast_kloop<TA>(A, a0, b0, lds, acc);
-
-    // Per-SNP class marginals of the tile, staged in LDS (the operand buffers
-    // are free once every wave has left the K loop).
-    __syncthreads();
-    int *mg_n = reinterpret_cast<int *>(lds);                     // [128][2]
-    double *mg_q = reinterpret_cast<double *>(lds + 1024);        // [128][2]
-    float *mg_f = reinterpret_cast<float *>(lds + 3072);          // [128][2]
+    // Per-SNP class marginals of the tile, staged in their own LDS region
+    // before the K loop, whose first barrier publishes them: their loads
+    // overlap the K loop's first stage instead of stalling the epilogue.
+    uint8_t *mgl = lds + fast_stage_lds<TA>();
+    int *mg_n = reinterpret_cast<int *>(mgl);                     // [128][2]
+    double *mg_q = reinterpret_cast<double *>(mgl + 1024);        // [128][2]
+    float *mg_f = reinterpret_cast<float *>(mgl + 3072);          // [128][2]
     if (tid < TA + TB) {
         int sn = tid < TA ? a0 + tid : b0 + tid - TA;
         sn = sn < A.nsnp ? sn : A.nsnp - 1;
@@ -527,7 +528,8 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
             mg_f[2 * tid + g] = (float)((double)mm.q[g] + oq[g]);
         }
     }
-    __syncthreads();
+    v16i acc[MB][4];
+    fast_kloop<TA>(A, a0, b0, lds, acc);
     const float sqV = (float)(A.sqV + A.ouSq), qVf = (float)(A.qVd + A.ouSum);
 
     // Outliers in the cells {0,1} x {0,1} of the lane's eight pairs, in one
@@ -585,68 +587,114 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
 
     // Epilogue: lane holds pairs (a = a0 + APW w + 4i + 2hf + p, b = b0 + 32bg + m);
     // acc[i][2bg+h][8p + 4g + w] = X[g][h][w].
+    // The counts and FP32 sums of pair (i, p, bg).
+    auto cells = [&](int i, int p, int bg, int n[9], float sf[9]) {
+        const int al = APW * wid + 4 * i + 2 * hf + p, bl = TA + 32 * bg + m;
+        int xc[3][3];
+        float xf[3][3];
 #pragma unroll
-    for (int i = 0; i < MB; i++) {
+        for (int g = 0; g < 2; g++) {
 #pragma unroll
-        for (int p = 0; p < 2; p++) {
-            const int al = APW * wid + 4 * i + 2 * hf + p;
-            const int a = a0 + al;
+            for (int h = 0; h < 2; h++) {
+                const v16i &v = acc[i][2 * bg + h];
+                const int e = 8 * p + 4 * g;
+                xc[g][h] = v[e];
+                xf[g][h] = (float)v[e + 1] + 256.0f * (float)v[e + 2] + 65536.0f * (float)v[e + 3] +
+                           corr[i][p][bg][2 * g + h];
+            }
+            xc[g][2] = mg_n[2 * al + g];
+            xf[g][2] = mg_f[2 * al + g];
+            xc[2][g] = mg_n[2 * bl + g];
+            xf[2][g] = mg_f[2 * bl + g];
+        }
+        xc[2][2] = A.nV;
+        xf[2][2] = qVf;
+        cells_from_x(xc, n);
+        cells_from_x(xf, sf);
+    };
+    // Pair (i, p, bg) through the screen, the FP64 decision and the hit list
+    // (or the dump); keep = false drops it. Every lane calls.
+    auto finish = [&](int i, int p, int bg, bool keep) {
+        const int al = APW * wid + 4 * i + 2 * hf + p, bl = TA + 32 * bg + m;
+        const int a = a0 + al, b = b0 + 32 * bg + m;
+        int n[9];
+        float sf[9];
+        cells(i, p, bg, n, sf);
+        // integer-valued doubles below 2^53: exact
+        auto exact = [&](double sd[9], double &sq) {
+            double xs[3][3];
 #pragma unroll
-            for (int bg = 0; bg < 2; bg++) {
-                const int bl = TA + 32 * bg + m;
-                const int b = b0 + 32 * bg + m;
-                int xc[3][3];
-                float xf[3][3];
+            for (int g = 0; g < 2; g++) {
 #pragma unroll
-                for (int g = 0; g < 2; g++) {
-#pragma unroll
-                    for (int h = 0; h < 2; h++) {
-                        const v16i &v = acc[i][2 * bg + h];
-                        const int e = 8 * p + 4 * g;
-                        xc[g][h] = v[e];
-                        xf[g][h] = (float)v[e + 1] + 256.0f * (float)v[e + 2] + 65536.0f * (float)v[e + 3] +
-                                   corr[i][p][bg][2 * g + h];
-                    }
-                    xc[g][2] = mg_n[2 * al + g];
-                    xf[g][2] = mg_f[2 * al + g];
-                    xc[2][g] = mg_n[2 * bl + g];
-                    xf[2][g] = mg_f[2 * bl + g];
+                for (int h = 0; h < 2; h++) {
+                    const v16i &v = acc[i][2 * bg + h];
+                    const int e = 8 * p + 4 * g;
+                    xs[g][h] = (double)v[e + 1] + 256.0 * (double)v[e + 2] + 65536.0 * (double)v[e + 3];
                 }
-                xc[2][2] = A.nV;
-                xf[2][2] = qVf;
-                const float sqf = sqV;
-                int n[9];
-                float sf[9];
-                cells_from_x(xc, n);
-                cells_from_x(xf, sf);
-                // integer-valued doubles below 2^53: exact
-                auto exact = [&](double sd[9], double &sq) {
-                    double xs[3][3];
+                xs[g][2] = mg_q[2 * al + g];
+                xs[2][g] = mg_q[2 * bl + g];
+            }
+            xs[2][2] = A.qVd + A.ouSum;
+            sq = A.sqV + A.ouSq;
+            if (A.nout) add_outlier_cells(A, a, b, xs);
+            cells_from_x(xs, sd);
+        };
+        const bool valid = keep && a < a1 && b < b1 && a > b;
+        finish_pair<DUMP>(hs, dsk, valid, a, b, n, sf, sqV, exact, CLS_IDENTITY, CLS_IDENTITY);
+    };
+    if constexpr (DUMP) {
+#pragma unroll
+        for (int i = 0; i < MB; i++)
+#pragma unroll
+            for (int p = 0; p < 2; p++)
+#pragma unroll
+                for (int bg = 0; bg < 2; bg++) finish(i, p, bg, true);
+    } else {
+        // Each pair through the FAST first stage of the screen (the scan's N, T
+        // and SST are the tile's); only when some lane of the wave keeps its
+        // pair does the wave take the per-pair path (full screen, FP64
+        // decision, hit append) for it.
+        const float Nf = (float)A.nV, T2N = qVf * qVf / Nf;
+#pragma unroll
+        for (int i = 0; i < MB; i++)
+#pragma unroll
+            for (int p = 0; p < 2; p++)
+#pragma unroll
+                for (int bg = 0; bg < 2; bg++) {
+                    const int al = APW * wid + 4 * i + 2 * hf + p, bl = TA + 32 * bg + m;
+                    const int a = a0 + al, b = b0 + 32 * bg + m;
+                    float xn[3][3], xf[3][3];
 #pragma unroll
                     for (int g = 0; g < 2; g++) {
 #pragma unroll
                         for (int h = 0; h < 2; h++) {
                             const v16i &v = acc[i][2 * bg + h];
                             const int e = 8 * p + 4 * g;
-                            xs[g][h] = (double)v[e + 1] + 256.0 * (double)v[e + 2] + 65536.0 * (double)v[e + 3];
+                            xn[g][h] = (float)v[e];
+                            xf[g][h] = (float)v[e + 1] + 256.0f * (float)v[e + 2] + 65536.0f * (float)v[e + 3] +
+                                       corr[i][p][bg][2 * g + h];
                         }
-                        xs[g][2] = mg_q[2 * al + g];
-                        xs[2][g] = mg_q[2 * bl + g];
+                        xn[g][2] = (float)mg_n[2 * al + g];
+                        xf[g][2] = mg_f[2 * al + g];
+                        xn[2][g] = (float)mg_n[2 * bl + g];
+                        xf[2][g] = mg_f[2 * bl + g];
                     }
-                    xs[2][2] = A.qVd + A.ouSum;
-                    sq = A.sqV + A.ouSq;
-                    if (A.nout) add_outlier_cells(A, a, b, xs);
-                    cells_from_x(xs, sd);
-                };
-                const bool valid = a < a1 && b < b1 && a > b;
-                finish_pair<DUMP>(hs, dsk, valid, a, b, n, sf, sqf, exact, CLS_IDENTITY, CLS_IDENTITY);
-            }
-        }
+                    xn[2][2] = Nf;
+                    xf[2][2] = qVf;
+                    float nf[9], sf[9];
+                    cells_from_x(xn, nf);
+                    cells_from_x(xf, sf);
+                    const bool keep = a < a1 && b < b1 && a > b &&
+                                      screen_fast_maybe(nf, sf, Nf, T2N, sqV, hs.test, hs.safe, hs.t1s, hs.t2s);
+                    if (__ballot(keep) != 0ull) finish(i, p, bg, keep);
+                    // keep the scheduler from interleaving the temporaries of several pairs
+                    __builtin_amdgcn_sched_barrier(0);
+                }
     }
 }
 
 template <int TA>
-constexpr int fast_lds() { return 2 * (2 * (TA + 64) + 5) * ROWB; }
+constexpr int fast_lds() { return fast_stage_lds<TA>() + FAST_MG_LDS; }
 
 template <int TA, bool DUMP>
 __global__ __launch_bounds__(fast_threads<TA>(), 64 / TA) void mfma_fast_kernel(const MfmaArgs A)
