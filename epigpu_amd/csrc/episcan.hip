@@ -1002,11 +1002,11 @@ int launch_pairs(epi_ctx *c, const epi_scan_params *p, const epi_schedule &sc, c
         if (ns && !d.sp_ok) return fail(c, EPI_ERR_STATE, "SPARSE tile data not built");
         if (sc.fast_rows == 64) {
             LAUNCH_TILES((mfma_fast_kernel<64, DUMP>), fast_threads<64>(), A, B.tiles, nf);
-            LAUNCH_TILES((mfma_sparse_kernel<64, DUMP>), fast_threads<64>(), A, B.tiles + nf, ns);
+            LAUNCH_TILES((mfma_sparse_kernel<64, DUMP>), sparse_threads<64>(), A, B.tiles + nf, ns);
             LAUNCH_TILES((mfma_general_kernel<64, DUMP>), general_threads<64>(), A, B.tiles + nf + ns, ng);
         } else {
             LAUNCH_TILES((mfma_fast_kernel<32, DUMP>), fast_threads<32>(), A, B.tiles, nf);
-            LAUNCH_TILES((mfma_sparse_kernel<32, DUMP>), fast_threads<32>(), A, B.tiles + nf, ns);
+            LAUNCH_TILES((mfma_sparse_kernel<32, DUMP>), sparse_threads<32>(), A, B.tiles + nf, ns);
             LAUNCH_TILES((mfma_general_kernel<32, DUMP>), general_threads<32>(), A, B.tiles + nf + ns, ng);
         }
     } else {
@@ -1189,6 +1189,15 @@ void drop_graph(epi_batch *b)
     b->gvalid = false;
 }
 
+// Threads per workgroup of batch_kernel(form, sc, kind).
+int batch_threads_of(int form, const epi_schedule &sc, int kind)
+{
+    if (form != EPI_FORM_MFMA) return 256;
+    if (sc.fast_rows == 64)
+        return kind == 2 ? batch_threads<64, 2>() : kind ? batch_threads<64, 1>() : batch_threads<64, 0>();
+    return kind == 2 ? batch_threads<32, 2>() : kind ? batch_threads<32, 1>() : batch_threads<32, 0>();
+}
+
 const void *batch_kernel(int form, const epi_schedule &sc, int kind)
 {
     if (form == EPI_FORM_MFMA) {
@@ -1216,9 +1225,9 @@ int capture_batch(epi_batch *b, const epi_scan_params *p)
     drop_graph(b);
     BatchArgs B[NKIND];
     const void *kfn[NKIND];
-    int grid[NKIND] = {0, 0, 0};
-    const int nt = p->form == EPI_FORM_MFMA ? 8 * b->work_sched.fast_rows : 256;
+    int grid[NKIND] = {0, 0, 0}, nt[NKIND];
     for (int kind = 0; kind < NKIND; kind++) {
+        nt[kind] = batch_threads_of(p->form, b->work_sched, kind);
         B[kind].mscan = b->d_mscan;
         B[kind].pscan = b->d_pscan;
         B[kind].work = b->d_work;
@@ -1227,7 +1236,7 @@ int capture_batch(epi_batch *b, const epi_scan_params *p)
         B[kind].hs = make_sink(b->hits, b->nhits, b->cap, p);
         kfn[kind] = batch_kernel(p->form, b->work_sched, kind);
         int per_cu = 1;
-        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn[kind], nt, 0));
+        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn[kind], nt[kind], 0));
         grid[kind] = std::max(1, per_cu) * c->ncu;
     }
     static const int korder[2][NKIND] = {{0, 2, 1}, {1, 2, 0}};
@@ -1239,7 +1248,7 @@ int capture_batch(epi_batch *b, const epi_scan_params *p)
         const int kind = ko[t];
         if (b->nkind[kind] == 0) continue;
         void *args[] = {&B[kind]};
-        le = hipLaunchKernel(kfn[kind], dim3(grid[kind]), dim3(nt), args, 0, c->stream);
+        le = hipLaunchKernel(kfn[kind], dim3(grid[kind]), dim3(nt[kind]), args, 0, c->stream);
     }
     hipError_t ce = hipStreamEndCapture(c->stream, &b->graph);
     if (le != hipSuccess) return fail(c, EPI_ERR_HIP, "launch during capture: %s", hipGetErrorString(le));

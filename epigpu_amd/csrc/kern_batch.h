@@ -54,14 +54,20 @@ __device__ __forceinline__ int next_item(const BatchArgs &B, int home)
     return -1;
 }
 
-// TA: FAST and SPARSE tiles are TA x 64 (TA = 32 or 64), GENERAL tiles
-// TA x 32; a workgroup has 8 TA threads for all three. KIND: the list this
-// kernel runs (0 FAST, 1 GENERAL, 2 SPARSE).
+// Threads of a workgroup of the batch kernel for tile kind KIND: the tile's own
+// (FAST 4 TA, one wave per SIMD; GENERAL and SPARSE 8 TA).
 template <int TA, int KIND>
-__global__ __launch_bounds__(fast_threads<TA>(), 64 / TA) void mfma_batch_kernel(const BatchArgs B)
+constexpr int batch_threads() { return KIND == 0 ? fast_threads<TA>() : KIND == 1 ? general_threads<TA>() : sparse_threads<TA>(); }
+template <int TA, int KIND>
+constexpr int batch_waves_per_simd() { return KIND == 0 ? fast_waves_per_simd<TA>() : 64 / TA; }
+
+// TA: FAST and SPARSE tiles are TA x 64 (TA = 32 or 64), GENERAL tiles
+// TA x 32. KIND: the list this kernel runs (0 FAST, 1 GENERAL, 2 SPARSE).
+template <int TA, int KIND>
+__global__ __launch_bounds__((batch_threads<TA, KIND>()), (batch_waves_per_simd<TA, KIND>())) void mfma_batch_kernel(
+    const BatchArgs B)
 {
     constexpr int GA = TA;
-    static_assert(general_threads<GA>() == fast_threads<TA>(), "one workgroup size for every tile kind");
     constexpr int LDS = KIND == 1 ? general_lds<GA>() : KIND == 2 ? sparse_lds<TA>() : fast_lds<TA>();
     __shared__ __attribute__((aligned(16))) uint8_t lds[LDS];
     __shared__ int item;

@@ -33,9 +33,24 @@
 // complete-case sum of squares.
 #pragma once
 
+#include <utility>
+
 #include "epilogue.h"
 
 namespace epi {
+
+// f(std::integral_constant<int, k>) for k = 0 .. N-1, as N separate calls:
+// every index is a compile-time constant, whatever the unroller decides.
+template <class F, int... I>
+__device__ __forceinline__ void static_for_seq(F &&f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    static_for_seq(f, std::make_integer_sequence<int, N>{});
+}
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
@@ -104,8 +119,10 @@ __device__ __forceinline__ uint32_t out_class(uint32_t code, int t) { return (co
 
 // FAST tiles (no missing call in the tile): every outlier is a complete case of
 // every pair, so only the four cells of classes {0,1} x {0,1} depend on the
-// pair. Adds sum_j u_j [g_a(j) = g][g_b(j) = h] to x[g][h] (FP64 path).
-__device__ __forceinline__ void add_outlier_cells(const MfmaScan &S, int a, int b, double x[3][3])
+// pair. Adds sum_j u_j [g_a(j) = g][g_b(j) = h] to x[g][h] (FP64 values for
+// the decision, FP32 for the screen).
+template <class T>
+__device__ __forceinline__ void add_outlier_cells(const MfmaScan &S, int a, int b, T x[3][3])
 {
     const int an = a < S.nsnp ? a : S.nsnp - 1, bn = b < S.nsnp ? b : S.nsnp - 1;
     for (int w = 0; 16 * w < S.nout; w++) {
@@ -113,12 +130,16 @@ __device__ __forceinline__ void add_outlier_cells(const MfmaScan &S, int a, int 
         const int nt = S.nout - 16 * w < 16 ? S.nout - 16 * w : 16;
         for (int t = 0; t < nt; t++) {
             const uint32_t ga = out_class(ca, t), gb = out_class(cb, t);
-            const double u = S.ov->d[16 * w + t];
-            const double u0 = ga == 0u ? u : 0.0, u1 = ga == 1u ? u : 0.0;
-            x[0][0] += gb == 0u ? u0 : 0.0;
-            x[0][1] += gb == 1u ? u0 : 0.0;
-            x[1][0] += gb == 0u ? u1 : 0.0;
-            x[1][1] += gb == 1u ? u1 : 0.0;
+            T u;
+            if constexpr (sizeof(T) == 8)
+                u = S.ov->d[16 * w + t];
+            else
+                u = S.ov->f[16 * w + t];
+            const T u0 = ga == 0u ? u : (T)0, u1 = ga == 1u ? u : (T)0;
+            x[0][0] += gb == 0u ? u0 : (T)0;
+            x[0][1] += gb == 1u ? u0 : (T)0;
+            x[1][0] += gb == 0u ? u1 : (T)0;
+            x[1][1] += gb == 1u ? u1 : (T)0;
         }
     }
 }
@@ -186,7 +207,13 @@ __device__ __forceinline__ v4i lds_v4(const uint8_t *p) { return *reinterpret_ca
 // 0xFF / 0x00 (A operands, which are ANDed with weight bytes). For a nibble
 // n, n * 0x00204081 = n + (n << 7) + (n << 14) + (n << 21) puts bit i of n
 // at bit 8i (the four shifted copies do not overlap), so masking with
-// 0x01010101 leaves one 0/1 per byte; (t << 8) - t widens each 1 to 0xFF.
+// 0x01010101 leaves one 0/1 per byte. The 0xFF form is a byte permute that
+// uses those 0/1 bytes as selectors into the constant 0x0000FF00 (byte 0 is
+// 0x00, byte 1 is 0xFF; both halves of the permute's 64-bit source hold it):
+// one full-rate instruction, where (t << 8) - t is folded by the compiler into
+// a quarter-rate 32-bit multiply.
+__device__ __forceinline__ uint32_t widen_ff(uint32_t t) { return __builtin_amdgcn_perm(0xFF00u, 0xFF00u, t); }
+
 template <bool FF>
 __device__ __forceinline__ v4i expand16(uint32_t x)
 {
@@ -194,7 +221,7 @@ __device__ __forceinline__ v4i expand16(uint32_t x)
 #pragma unroll
     for (int d = 0; d < 4; d++) {
         const uint32_t t = (((x >> (4 * d)) & 0xFu) * 0x00204081u) & 0x01010101u;
-        r[d] = (int)(FF ? (t << 8) - t : t);
+        r[d] = (int)(FF ? widen_ff(t) : t);
     }
     return r;
 }
@@ -205,7 +232,7 @@ __device__ __forceinline__ v4i expand16_sel(uint32_t x, bool ff)
 #pragma unroll
     for (int d = 0; d < 4; d++) {
         const uint32_t t = (((x >> (4 * d)) & 0xFu) * 0x00204081u) & 0x01010101u;
-        r[d] = (int)(ff ? (t << 8) - t : t);
+        r[d] = (int)(ff ? widen_ff(t) : t);
     }
     return r;
 }
@@ -233,6 +260,9 @@ struct Stager {
     static_assert(NB * NT >= NDW && (NB - 1) * NT < NDW, "dwords per thread");
     static_assert(NWR % 4 == 0 && NWR * KC / 8 == 64 * (NWR / 4), "weight rows per wave");
     static constexpr bool PART = NB * NT > NDW;         // the last dword round leaves lanes idle
+    static constexpr int NBD = NB;                      // dwords per thread per stage
+    static constexpr int NLD = NB + 1;                  // global loads per stage (dwords, weights)
+    static constexpr int NST = 2 * NB + 1;              // LDS writes per stage
     static_assert((NDW - (NB - 1) * NT) % 64 == 0, "idle lanes fill whole waves");
     bool pact;                                          // this wave has a dword in the last round
     const uint32_t *src[NB];
@@ -281,11 +311,12 @@ struct Stager {
             if (!PART || i < NB - 1 || pact) v[S][i] = src[i][off];
         w[S] = *reinterpret_cast<const int2 *>(wsrc + k0);
     }
-    template <int S>
+    // dwords [I0, I1) of register set S, and the weight piece when W
+    template <int S, int I0 = 0, int I1 = NB, bool W = true>
     __device__ __forceinline__ void store(uint8_t *st) const
     {
 #pragma unroll
-        for (int i = 0; i < NB; i++) {
+        for (int i = I0; i < I1; i++) {
             if (PART && i == NB - 1 && !pact) continue;
             v4i lo, hi;
             if (NT * (i + 1) <= NA) {
@@ -301,7 +332,7 @@ struct Stager {
             *reinterpret_cast<v4i *>(st + dst[i]) = lo;
             *reinterpret_cast<v4i *>(st + dst[i] + 16) = hi;
         }
-        *reinterpret_cast<int2 *>(st + wdst) = w[S];
+        if (W) *reinterpret_cast<int2 *>(st + wdst) = w[S];
     }
 };
 
@@ -325,36 +356,44 @@ __device__ __forceinline__ void k_loop(St &sg, uint8_t *lds, int stage_bytes, in
     sg.template store<0>(lds);
     sg.template load<0>(2 * KC < Kl ? 2 * KC : 0);
     __syncthreads();
-    // One stage per region between barriers: step 3 of the previous stage
-    // (o1; zero operands before the first stage) and steps 0-2 of this one
-    // are issued while this stage's operands are read from LDS and the next
-    // stage is expanded into the other buffer.
+    // One stage per region between barriers, in four steps. Step t issues
+    // the MFMAs of the operands read during step t - 1 (step 0 those of the
+    // previous stage's step 3; zero operands before the first stage), reads
+    // the next step's operands from LDS, and does a quarter of the wave's
+    // staging: the first and second halves of the next stage's expansion
+    // and LDS writes, then the global loads of the stage after it. A
+    // scheduling fence closes each step, so that the compiler cannot hoist
+    // the next step's ANDs (whose operands are still in flight from LDS)
+    // into this step's MFMAs; within a step, each MFMA is followed by up to
+    // three VALU instructions and one LDS read.
+    // Masks: 0x008 MFMA, 0x002 VALU, 0x100 LDS read.
+    constexpr int H = (St::NBD + 1) / 2;
+    auto step_end = [&]() {
+#pragma unroll
+        for (int u = 0; u < NMF; u++) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    };
     auto body = [&](auto tag, const uint8_t *cur, uint8_t *nxt, int kc) {
         constexpr int S = decltype(tag)::v;
+        issue(o1);
         fetch(cur, 0, o0);
-        issue(o1);
+        sg.template store<S, 0, H, false>(nxt);
+        step_end();
+        issue(o0);
         fetch(cur, 1, o1);
-        issue(o0);
-        fetch(cur, 2, o0);
-        sg.template store<S>(nxt);
-        sg.template load<S>(kc + 3 * KC < Kl ? kc + 3 * KC : Kl - KC);
+        sg.template store<S, H, St::NBD, true>(nxt);
+        step_end();
         issue(o1);
-        fetch(cur, 3, o1);
+        fetch(cur, 2, o0);
+        sg.template load<S>(kc + 3 * KC < Kl ? kc + 3 * KC : Kl - KC);
+        step_end();
         issue(o0);
-        // Instruction order of the stage: after each MFMA up to five VALU
-        // instructions (expansion, AND) and one LDS read, an LDS write every
-        // fourth MFMA and the global loads after the fourth, so that the
-        // MFMA pipe stays fed while the wave does its share of staging.
-        // Masks: 0x008 MFMA, 0x002 VALU, 0x100 LDS read, 0x200 LDS write,
-        // 0x020 global read.
-#pragma unroll
-        for (int u = 0; u < 4 * NMF; u++) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-            __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);
-            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-            if (u % 4 == 2) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
-            if (u >= 4 && u < 8) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
-        }
+        fetch(cur, 3, o1);
+        step_end();
         // the next stage was written by every wave during this one
         __syncthreads();
     };
@@ -400,8 +439,8 @@ __device__ __forceinline__ void k_loop1(St &sg, uint8_t *lds, int stage_bytes, i
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
             __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);
             __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-            if (u % 4 == 2) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
-            if (u >= 4 && u < 8) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+            if (u % 4 == 2 && u / 4 < St::NST) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+            if (u >= 4 && u < 4 + St::NLD) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
         }
         // the next stage was written by every wave during this one
         __syncthreads();
@@ -418,26 +457,38 @@ __device__ __forceinline__ void k_loop1(St &sg, uint8_t *lds, int stage_bytes, i
 }
 
 // ---------------------------------------------------------------- FAST ----
-// Tile TA a x 64 b (TA = 32 or 64) with TA / 8 waves; wave w owns a-SNPs
-// [8w, 8w+8) x all 64 b as two blocks of 32 A rows: 8 MFMAs per
-// 32-individual step and 128 accumulator registers per lane, so a SIMD runs
-// two waves (two 256-thread workgroups, or one of 512 threads, per CU).
+// Tile TA a x 64 b (TA = 32 or 64). Each wave owns APW = 4 MB a-SNPs x all
+// 64 b as MB blocks of 32 A rows (4 a-SNPs x 2 classes x 4 weights): 4 MB
+// MFMAs per 32-individual step and 64 MB accumulator registers per lane.
+// FAST tiles take MB = 4: 16 a-SNPs per wave, TA / 16 waves, 256 accumulators
+// (AGPRs) per lane and one wave per SIMD, whose 512 registers leave room for
+// two operand sets in flight; 9 LDS operand reads per 16 MFMAs. SPARSE tiles
+// take MB = 2 (8 a-SNPs per wave, TA / 8 waves, two waves per SIMD), since
+// their missing-call passes need the registers of a second wave's share.
 // LDS rows per stage: A [plane 0..1][TA a], B [plane 0..1][64 b],
 // W [V, L0, L1, L2], a dummy row. Two stage buffers; operands of the next 32-individual
 // step are read from LDS while the MFMAs of the current step run.
+template <int TA, int MB>
+constexpr int kloop_threads() { return 64 * TA / (4 * MB); }
+constexpr int FAST_MB = 2;  // A blocks per wave of a FAST tile
 template <int TA>
-constexpr int fast_threads() { return 8 * TA; }
+constexpr int fast_threads() { return kloop_threads<TA, FAST_MB>(); }
+// waves per SIMD that a FAST tile's registers allow
+template <int TA>
+constexpr int fast_waves_per_simd() { return FAST_MB == 4 ? 1 : 64 / TA; }
+template <int TA>
+constexpr int sparse_threads() { return kloop_threads<TA, 2>(); }
 
-// The K loop of a FAST tile: acc[i][2bg + h][8p + 4g + w] = X[g][h][w] of
-// the pair (a0 + 8 wid + 4i + 2hf + p, b0 + 32bg + m), m = lane & 31, hf = lane >> 5,
+// The K loop of a tile: acc[i][2bg + h][8p + 4g + w] = X[g][h][w] of the
+// pair (a0 + 4MB wid + 4i + 2hf + p, b0 + 32bg + m), m = lane & 31, hf = lane >> 5,
 // for classes g, h in {0, 1} and weights w = V, L0, L1, L2.
-template <int TA>
-__device__ __forceinline__ void fast_kloop(const MfmaScan &A, int a0, int b0, uint8_t *lds, v16i (&acc)[2][4])
+template <int TA, int MB>
+__device__ __forceinline__ void fast_kloop(const MfmaScan &A, int a0, int b0, uint8_t *lds, v16i (&acc)[MB][4])
 {
     static_assert(TA == 64 || TA == 32, "fast tile height");
-    constexpr int TB = 64, NPL = 2, NT = fast_threads<TA>();
-    constexpr int MB = 2;                     // 32-row A blocks per wave
-    constexpr int APW = 8;                    // a-SNPs per wave
+    static_assert(MB == 2 || MB == 4, "A blocks per wave");
+    constexpr int TB = 64, NPL = 2, NT = kloop_threads<TA, MB>();
+    constexpr int APW = 4 * MB;               // a-SNPs per wave
     constexpr int ROWS_AB = NPL * (TA + TB);  // 192 or 256
     constexpr int NCH = ROWS_AB * (KC / 32) / NT;  // bitplane dwords per thread per stage
     constexpr int STAGE = (ROWS_AB + 5) * ROWB;
@@ -488,14 +539,14 @@ __device__ __forceinline__ void fast_kloop(const MfmaScan &A, int a0, int b0, ui
         }
     };
 
-    k_loop<decltype(sg), Ops, 2 * 4>(sg, lds, STAGE, A.Kl, fetch, issue);
+    k_loop<decltype(sg), Ops, 4 * MB>(sg, lds, STAGE, A.Kl, fetch, issue);
 }
 
 template <int TA, bool DUMP>
 __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink &hs, const DumpSink &dsk, int a0,
                                                int a1, int b0, int b1, uint8_t *lds)
 {
-    constexpr int TB = 64, MB = 2, APW = 8;
+    constexpr int TB = 64, MB = FAST_MB, APW = 4 * MB;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int m = lane & 31, hf = lane >> 5;
     // Per-SNP class marginals of the tile, staged in their own LDS region
@@ -529,108 +580,79 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
         }
     }
     v16i acc[MB][4];
-    fast_kloop<TA>(A, a0, b0, lds, acc);
+    fast_kloop<TA, MB>(A, a0, b0, lds, acc);
     const float sqV = (float)(A.sqV + A.ouSq), qVf = (float)(A.qVd + A.ouSum);
 
-    // Outliers in the cells {0,1} x {0,1} of the lane's eight pairs, in one
-    // wave-uniform loop: corr[i][p][bg][2g+h].
-    float corr[MB][2][2][4];
-#pragma unroll
-    for (int i = 0; i < MB; i++)
-#pragma unroll
-        for (int p = 0; p < 2; p++)
-#pragma unroll
-            for (int bg = 0; bg < 2; bg++)
-#pragma unroll
-                for (int c = 0; c < 4; c++) corr[i][p][bg][c] = 0.0f;
-    if (A.nout) {
-        for (int w = 0; 16 * w < A.nout; w++) {
-            uint32_t ca[MB][2], cb[2];
-#pragma unroll
-            for (int i = 0; i < MB; i++)
-#pragma unroll
-                for (int p = 0; p < 2; p++) {
-                    const int a = a0 + APW * wid + 4 * i + 2 * hf + p;
-                    ca[i][p] = A.ocodes[(size_t)w * A.nsnp + (a < A.nsnp ? a : A.nsnp - 1)];
-                }
-#pragma unroll
-            for (int bg = 0; bg < 2; bg++) {
-                const int b = b0 + 32 * bg + m;
-                cb[bg] = A.ocodes[(size_t)w * A.nsnp + (b < A.nsnp ? b : A.nsnp - 1)];
-            }
-            const int nt = A.nout - 16 * w < 16 ? A.nout - 16 * w : 16;
-            for (int t = 0; t < nt; t++) {
-                const float u = A.ov->f[16 * w + t];
-                float ub[2][2];
-#pragma unroll
-                for (int bg = 0; bg < 2; bg++) {
-                    const uint32_t gb = out_class(cb[bg], t);
-                    ub[bg][0] = gb == 0u ? u : 0.0f;
-                    ub[bg][1] = gb == 1u ? u : 0.0f;
-                }
-#pragma unroll
-                for (int i = 0; i < MB; i++)
-#pragma unroll
-                    for (int p = 0; p < 2; p++) {
-                        const uint32_t ga = out_class(ca[i][p], t);
-#pragma unroll
-                        for (int bg = 0; bg < 2; bg++)
-#pragma unroll
-                            for (int h = 0; h < 2; h++) {
-                                corr[i][p][bg][h] += ga == 0u ? ub[bg][h] : 0.0f;
-                                corr[i][p][bg][2 + h] += ga == 1u ? ub[bg][h] : 0.0f;
-                            }
-                    }
-            }
-        }
-    }
-
     // Epilogue: lane holds pairs (a = a0 + APW w + 4i + 2hf + p, b = b0 + 32bg + m);
-    // acc[i][2bg+h][8p + 4g + w] = X[g][h][w].
-    // The counts and FP32 sums of pair (i, p, bg).
-    auto cells = [&](int i, int p, int bg, int n[9], float sf[9]) {
-        const int al = APW * wid + 4 * i + 2 * hf + p, bl = TA + 32 * bg + m;
+    // acc[i][2bg+h][8p + 4g + w] = X[g][h][w]. A pair's values are copied out
+    // of the accumulators with constant indices (FastX), so that the per-pair
+    // path, however the compiler inlines it, never indexes the accumulators.
+    struct FastX {
+        int n[2][2], l[2][2][3];  // counts and limb sums of the cells {0,1} x {0,1}
+        int al, bl;               // the pair's a and b rows in the tile marginals
+    };
+    auto pair_x = [&](int i, int p, int bg) {
+        FastX x;
+#pragma unroll
+        for (int g = 0; g < 2; g++)
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const v16i &v = acc[i][2 * bg + h];
+                const int e = 8 * p + 4 * g;
+                x.n[g][h] = v[e];
+                x.l[g][h][0] = v[e + 1];
+                x.l[g][h][1] = v[e + 2];
+                x.l[g][h][2] = v[e + 3];
+            }
+        x.al = APW * wid + 4 * i + 2 * hf + p;
+        x.bl = TA + 32 * bg + m;
+        return x;
+    };
+    // FP32 sums of the pair's 3 x 3 table: the limbs, the outliers of the
+    // cells {0,1} x {0,1} (all of them complete cases in a FAST tile) and the
+    // marginals.
+    auto sums_f = [&](const FastX &x, float xf[3][3]) {
+#pragma unroll
+        for (int g = 0; g < 2; g++)
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+                xf[g][h] = (float)x.l[g][h][0] + 256.0f * (float)x.l[g][h][1] + 65536.0f * (float)x.l[g][h][2];
+        if (A.nout) add_outlier_cells(A, a0 + x.al, b0 + x.bl - TA, xf);
+#pragma unroll
+        for (int g = 0; g < 2; g++) {
+            xf[g][2] = mg_f[2 * x.al + g];
+            xf[2][g] = mg_f[2 * x.bl + g];
+        }
+        xf[2][2] = qVf;
+    };
+    // Pair x through the screen, the FP64 decision and the hit list (or the
+    // dump); keep = false drops it. Every lane calls.
+    auto finish = [&](const FastX &x, bool keep) __attribute__((always_inline)) {
+        const int al = x.al, bl = x.bl;
+        const int a = a0 + al, b = b0 + bl - TA;
         int xc[3][3];
         float xf[3][3];
 #pragma unroll
         for (int g = 0; g < 2; g++) {
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const v16i &v = acc[i][2 * bg + h];
-                const int e = 8 * p + 4 * g;
-                xc[g][h] = v[e];
-                xf[g][h] = (float)v[e + 1] + 256.0f * (float)v[e + 2] + 65536.0f * (float)v[e + 3] +
-                           corr[i][p][bg][2 * g + h];
-            }
+            for (int h = 0; h < 2; h++) xc[g][h] = x.n[g][h];
             xc[g][2] = mg_n[2 * al + g];
-            xf[g][2] = mg_f[2 * al + g];
             xc[2][g] = mg_n[2 * bl + g];
-            xf[2][g] = mg_f[2 * bl + g];
         }
         xc[2][2] = A.nV;
-        xf[2][2] = qVf;
-        cells_from_x(xc, n);
-        cells_from_x(xf, sf);
-    };
-    // Pair (i, p, bg) through the screen, the FP64 decision and the hit list
-    // (or the dump); keep = false drops it. Every lane calls.
-    auto finish = [&](int i, int p, int bg, bool keep) {
-        const int al = APW * wid + 4 * i + 2 * hf + p, bl = TA + 32 * bg + m;
-        const int a = a0 + al, b = b0 + 32 * bg + m;
+        sums_f(x, xf);
         int n[9];
         float sf[9];
-        cells(i, p, bg, n, sf);
+        cells_from_x(xc, n);
+        cells_from_x(xf, sf);
         // integer-valued doubles below 2^53: exact
         auto exact = [&](double sd[9], double &sq) {
             double xs[3][3];
 #pragma unroll
             for (int g = 0; g < 2; g++) {
 #pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    const v16i &v = acc[i][2 * bg + h];
-                    const int e = 8 * p + 4 * g;
-                    xs[g][h] = (double)v[e + 1] + 256.0 * (double)v[e + 2] + 65536.0 * (double)v[e + 3];
-                }
+                for (int h = 0; h < 2; h++)
+                    xs[g][h] = (double)x.l[g][h][0] + 256.0 * (double)x.l[g][h][1] + 65536.0 * (double)x.l[g][h][2];
                 xs[g][2] = mg_q[2 * al + g];
                 xs[2][g] = mg_q[2 * bl + g];
             }
@@ -642,54 +664,41 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
         const bool valid = keep && a < a1 && b < b1 && a > b;
         finish_pair<DUMP>(hs, dsk, valid, a, b, n, sf, sqV, exact, CLS_IDENTITY, CLS_IDENTITY);
     };
+    // pair slot k = (i, p, bg) = (k / 4, k / 2 % 2, k % 2)
     if constexpr (DUMP) {
-#pragma unroll
-        for (int i = 0; i < MB; i++)
-#pragma unroll
-            for (int p = 0; p < 2; p++)
-#pragma unroll
-                for (int bg = 0; bg < 2; bg++) finish(i, p, bg, true);
+        static_for<4 * MB>([&](auto k) {
+            constexpr int K = decltype(k)::value;
+            finish(pair_x(K / 4, K / 2 % 2, K % 2), true);
+        });
     } else {
         // Each pair through the FAST first stage of the screen (the scan's N, T
         // and SST are the tile's); only when some lane of the wave keeps its
         // pair does the wave take the per-pair path (full screen, FP64
         // decision, hit append) for it.
         const float Nf = (float)A.nV, T2N = qVf * qVf / Nf;
-#pragma unroll
-        for (int i = 0; i < MB; i++)
-#pragma unroll
-            for (int p = 0; p < 2; p++)
-#pragma unroll
-                for (int bg = 0; bg < 2; bg++) {
-                    const int al = APW * wid + 4 * i + 2 * hf + p, bl = TA + 32 * bg + m;
-                    const int a = a0 + al, b = b0 + 32 * bg + m;
+        static_for<4 * MB>([&](auto k) {
+                    constexpr int K = decltype(k)::value, bg = K % 2;
+                    const FastX x = pair_x(K / 4, K / 2 % 2, bg);
+                    const int a = a0 + x.al, b = b0 + 32 * bg + m;
                     float xn[3][3], xf[3][3];
 #pragma unroll
                     for (int g = 0; g < 2; g++) {
 #pragma unroll
-                        for (int h = 0; h < 2; h++) {
-                            const v16i &v = acc[i][2 * bg + h];
-                            const int e = 8 * p + 4 * g;
-                            xn[g][h] = (float)v[e];
-                            xf[g][h] = (float)v[e + 1] + 256.0f * (float)v[e + 2] + 65536.0f * (float)v[e + 3] +
-                                       corr[i][p][bg][2 * g + h];
-                        }
-                        xn[g][2] = (float)mg_n[2 * al + g];
-                        xf[g][2] = mg_f[2 * al + g];
-                        xn[2][g] = (float)mg_n[2 * bl + g];
-                        xf[2][g] = mg_f[2 * bl + g];
+                        for (int h = 0; h < 2; h++) xn[g][h] = (float)x.n[g][h];
+                        xn[g][2] = (float)mg_n[2 * x.al + g];
+                        xn[2][g] = (float)mg_n[2 * x.bl + g];
                     }
                     xn[2][2] = Nf;
-                    xf[2][2] = qVf;
+                    sums_f(x, xf);
                     float nf[9], sf[9];
                     cells_from_x(xn, nf);
                     cells_from_x(xf, sf);
                     const bool keep = a < a1 && b < b1 && a > b &&
                                       screen_fast_maybe(nf, sf, Nf, T2N, sqV, hs.test, hs.safe, hs.t1s, hs.t2s);
-                    if (__ballot(keep) != 0ull) finish(i, p, bg, keep);
+                    if (__ballot(keep) != 0ull) finish(x, keep);
                     // keep the scheduler from interleaving the temporaries of several pairs
                     __builtin_amdgcn_sched_barrier(0);
-                }
+        });
     }
 }
 
@@ -697,7 +706,7 @@ template <int TA>
 constexpr int fast_lds() { return fast_stage_lds<TA>() + FAST_MG_LDS; }
 
 template <int TA, bool DUMP>
-__global__ __launch_bounds__(fast_threads<TA>(), 64 / TA) void mfma_fast_kernel(const MfmaArgs A)
+__global__ __launch_bounds__(fast_threads<TA>(), fast_waves_per_simd<TA>()) void mfma_fast_kernel(const MfmaArgs A)
 {
     __shared__ __attribute__((aligned(16))) uint8_t lds[fast_lds<TA>()];
     const int4 T = A.tiles[xcd_tile(blockIdx.x, A.ntiles)];
@@ -917,7 +926,7 @@ __device__ __forceinline__ void mfma_sparse_tile(const MfmaScan &A, const HitSin
     const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int m = lane & 31, hf = lane >> 5;
     v16i acc[MB][4];
-    fast_kloop<TA>(A, a0, b0, lds, acc);
+    fast_kloop<TA, MB>(A, a0, b0, lds, acc);
     __syncthreads();  // the stage buffers are free
 
     // Marginals of the tile's SNPs (a-SNPs first), pass-2 and pass-1 values.
@@ -1073,7 +1082,7 @@ __device__ __forceinline__ void mfma_sparse_tile(const MfmaScan &A, const HitSin
 }
 
 template <int TA, bool DUMP>
-__global__ __launch_bounds__(fast_threads<TA>(), 64 / TA) void mfma_sparse_kernel(const MfmaArgs A)
+__global__ __launch_bounds__(sparse_threads<TA>(), 64 / TA) void mfma_sparse_kernel(const MfmaArgs A)
 {
     __shared__ __attribute__((aligned(16))) uint8_t lds[sparse_lds<TA>()];
     const int4 T = A.tiles[xcd_tile(blockIdx.x, A.ntiles)];
