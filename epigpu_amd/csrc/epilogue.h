@@ -241,6 +241,22 @@ __device__ __forceinline__ bool screen_fast_maybe(const float n[9], const float 
     return dfok && (guard || f1 || f2);
 }
 
+// screen_fast_maybe for a pair whose nine cells are all occupied (nfac = 9,
+// d1 = 8, d2 = N - 9 >= 1, which the caller checks): with c1 = 8 t1s / d2 and
+// c2 = 4 t2s / d2 (c2 < 0 for test f, so that its test never holds), F > t1s
+// is ssb d2 > 8 t1s ssw, i.e. ssb > c1 ssw (d2 > 0), and the F2 bound is ssb >
+// c2 ssw; SST = sq - T^2 / N. The same decisions as screen_fast_maybe.
+__device__ __forceinline__ bool screen_fast9(const float n[9], const float s[9], float T2N, float SST, float sq,
+                                             float c1, float c2)
+{
+    float ssb = -T2N;
+#pragma unroll
+    for (int c = 0; c < 9; c++) ssb = __builtin_fmaf(s[c] * s[c], __builtin_amdgcn_rcpf(n[c]), ssb);
+    const float ssw = SST - ssb;
+    const bool guard = !(ssw > 1e-4f * (sq + T2N + ssb));
+    return guard | (ssb > c1 * ssw) | ((c2 >= 0.0f) & (ssb > c2 * ssw));
+}
+
 // Wave-aggregated append: one atomic per wave and call. Every lane of the
 // wave must call it (hit may be false).
 __device__ __forceinline__ void append_hit(const HitSink &hs, bool hit, const DevHit &h)

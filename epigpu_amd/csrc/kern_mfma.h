@@ -676,6 +676,13 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
         // pair does the wave take the per-pair path (full screen, FP64
         // decision, hit append) for it.
         const float Nf = (float)A.nV, T2N = qVf * qVf / Nf;
+        // Pairs with all nine cells occupied (nearly all of them) take
+        // screen_fast9, whose constants are the scan's; a wave with any other
+        // pair takes screen_fast_maybe.
+        const float d2_9 = Nf - 9.0f;
+        const bool all9_ok = d2_9 >= 1.0f;
+        const float c1 = 8.0f * hs.t1s / d2_9, c2 = hs.test != 'f' ? 4.0f * hs.t2s / d2_9 : -1.0f;
+        const float SST = sqV - T2N;
         static_for<4 * MB>([&](auto k) {
                     constexpr int K = decltype(k)::value, bg = K % 2;
                     const FastX x = pair_x(K / 4, K / 2 % 2, bg);
@@ -693,8 +700,15 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
                     float nf[9], sf[9];
                     cells_from_x(xn, nf);
                     cells_from_x(xf, sf);
-                    const bool keep = a < a1 && b < b1 && a > b &&
-                                      screen_fast_maybe(nf, sf, Nf, T2N, sqV, hs.test, hs.safe, hs.t1s, hs.t2s);
+                    const bool valid = (a < a1) & (b < b1) & (a > b);
+                    float nmin = nf[0];
+#pragma unroll
+                    for (int c = 1; c < 9; c++) nmin = fminf(nmin, nf[c]);
+                    bool keep;
+                    if (all9_ok && __ballot(!(nmin > 0.0f)) == 0ull)
+                        keep = valid & screen_fast9(nf, sf, T2N, SST, sqV, c1, c2);
+                    else
+                        keep = valid & screen_fast_maybe(nf, sf, Nf, T2N, sqV, hs.test, hs.safe, hs.t1s, hs.t2s);
                     if (__ballot(keep) != 0ull) finish(x, keep);
                     // keep the scheduler from interleaving the temporaries of several pairs
                     __builtin_amdgcn_sched_barrier(0);
