@@ -365,14 +365,17 @@ __device__ __forceinline__ void k_loop(St &sg, uint8_t *lds, int stage_bytes, in
     // scheduling fence closes each step, so that the compiler cannot hoist
     // the next step's ANDs (whose operands are still in flight from LDS)
     // into this step's MFMAs; within a step, each MFMA is followed by up to
-    // three VALU instructions and one LDS read.
+    // VS VALU instructions and one LDS read. VS follows the staging load:
+    // three for two dwords per thread and stage (64-row tiles), four for three
+    // (32-row tiles), the fastest of 2 to 5 for each (c3 and c4).
     // Masks: 0x008 MFMA, 0x002 VALU, 0x100 LDS read.
     constexpr int H = (St::NBD + 1) / 2;
+    constexpr int VS = St::NBD >= 3 ? 4 : 3;
     auto step_end = [&]() {
 #pragma unroll
         for (int u = 0; u < NMF; u++) {
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-            __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, VS, 0);
             __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
         }
         __builtin_amdgcn_sched_barrier(0);
