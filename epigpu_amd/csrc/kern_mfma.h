@@ -311,6 +311,27 @@ struct Stager {
             if (!PART || i < NB - 1 || pact) v[S][i] = src[i][off];
         w[S] = *reinterpret_cast<const int2 *>(wsrc + k0);
     }
+    // half-dwords [H0, H1) of register set S (half h: 16 bits of dword h / 2,
+    // the low half first, 16 bytes of its LDS row), and the weight piece when W
+    template <int S, int H0, int H1, bool W>
+    __device__ __forceinline__ void store_halves(uint8_t *st) const
+    {
+#pragma unroll
+        for (int h = H0; h < H1; h++) {
+            const int i = h / 2;
+            if (PART && i == NB - 1 && !pact) continue;
+            const uint32_t x = h % 2 ? v[S][i] >> 16 : v[S][i] & 0xFFFFu;
+            v4i r;
+            if (NT * (i + 1) <= NA)
+                r = expand16<true>(x);
+            else if (NT * i >= NA)
+                r = expand16<false>(x);
+            else
+                r = expand16_sel(x, isa[i]);
+            *reinterpret_cast<v4i *>(st + dst[i] + 16 * (h % 2)) = r;
+        }
+        if (W) *reinterpret_cast<int2 *>(st + wdst) = w[S];
+    }
     // dwords [I0, I1) of register set S, and the weight piece when W
     template <int S, int I0 = 0, int I1 = NB, bool W = true>
     __device__ __forceinline__ void store(uint8_t *st) const
@@ -360,8 +381,12 @@ __device__ __forceinline__ void k_loop(St &sg, uint8_t *lds, int stage_bytes, in
     // the MFMAs of the operands read during step t - 1 (step 0 those of the
     // previous stage's step 3; zero operands before the first stage), reads
     // the next step's operands from LDS, and does a quarter of the wave's
-    // staging: the first and second halves of the next stage's expansion
-    // and LDS writes, then the global loads of the stage after it. A
+    // staging. Two dwords per thread (64-row tiles): the first and second
+    // halves of the next stage's expansion and LDS writes in steps 0 and 1,
+    // the global loads of the stage after it in step 2. Three or more (32-row
+    // tiles): the expansion and writes by 16-bit halves spread over all four
+    // steps, the global loads after the last write. (Each layout measured
+    // faster than the other on its tiles: c3 and c4.) A
     // scheduling fence closes each step, so that the compiler cannot hoist
     // the next step's ANDs (whose operands are still in flight from LDS)
     // into this step's MFMAs; within a step, each MFMA is followed by up to
@@ -382,21 +407,42 @@ __device__ __forceinline__ void k_loop(St &sg, uint8_t *lds, int stage_bytes, in
     };
     auto body = [&](auto tag, const uint8_t *cur, uint8_t *nxt, int kc) {
         constexpr int S = decltype(tag)::v;
-        issue(o1);
-        fetch(cur, 0, o0);
-        sg.template store<S, 0, H, false>(nxt);
-        step_end();
-        issue(o0);
-        fetch(cur, 1, o1);
-        sg.template store<S, H, St::NBD, true>(nxt);
-        step_end();
-        issue(o1);
-        fetch(cur, 2, o0);
-        sg.template load<S>(kc + 3 * KC < Kl ? kc + 3 * KC : Kl - KC);
-        step_end();
-        issue(o0);
-        fetch(cur, 3, o1);
-        step_end();
+        if constexpr (St::NBD >= 3) {
+            constexpr int NH = 2 * St::NBD;  // half-dwords per thread
+            issue(o1);
+            fetch(cur, 0, o0);
+            sg.template store_halves<S, 0, NH / 4, false>(nxt);
+            step_end();
+            issue(o0);
+            fetch(cur, 1, o1);
+            sg.template store_halves<S, NH / 4, NH / 2, false>(nxt);
+            step_end();
+            issue(o1);
+            fetch(cur, 2, o0);
+            sg.template store_halves<S, NH / 2, 3 * NH / 4, false>(nxt);
+            step_end();
+            issue(o0);
+            fetch(cur, 3, o1);
+            sg.template store_halves<S, 3 * NH / 4, NH, true>(nxt);
+            sg.template load<S>(kc + 3 * KC < Kl ? kc + 3 * KC : Kl - KC);
+            step_end();
+        } else {
+            issue(o1);
+            fetch(cur, 0, o0);
+            sg.template store<S, 0, H, false>(nxt);
+            step_end();
+            issue(o0);
+            fetch(cur, 1, o1);
+            sg.template store<S, H, St::NBD, true>(nxt);
+            step_end();
+            issue(o1);
+            fetch(cur, 2, o0);
+            sg.template load<S>(kc + 3 * KC < Kl ? kc + 3 * KC : Kl - KC);
+            step_end();
+            issue(o0);
+            fetch(cur, 3, o1);
+            step_end();
+        }
         // the next stage was written by every wave during this one
         __syncthreads();
     };
