@@ -386,13 +386,14 @@ __device__ __forceinline__ void k_loop(St &sg, uint8_t *lds, int stage_bytes, in
     // the global loads of the stage after it in step 2. Three or more (32-row
     // tiles): the expansion and writes by 16-bit halves spread over all four
     // steps, the global loads after the last write. (Each layout measured
-    // faster than the other on its tiles: c3 and c4.) A
-    // scheduling fence closes each step, so that the compiler cannot hoist
-    // the next step's ANDs (whose operands are still in flight from LDS)
-    // into this step's MFMAs; within a step, each MFMA is followed by up to
-    // VS VALU instructions and one LDS read. VS follows the staging load:
+    // faster than the other on its tiles: a scan of N = 8000, M = 50000 on
+    // 64-row tiles, a batch of 64 scans with N from 300 to 6000 on 32-row
+    // tiles.) A scheduling fence closes each step, so that the compiler
+    // cannot hoist the next step's ANDs (whose operands are still in flight
+    // from LDS) into this step's MFMAs; within a step, each MFMA is followed
+    // by up to VS VALU instructions and one LDS read. VS follows the staging load:
     // three for two dwords per thread and stage (64-row tiles), four for three
-    // (32-row tiles), the fastest of 2 to 5 for each (c3 and c4).
+    // (32-row tiles), the fastest of 2 to 5 for each on the same two workloads.
     // Masks: 0x008 MFMA, 0x002 VALU, 0x100 LDS read.
     constexpr int H = (St::NBD + 1) / 2;
     constexpr int VS = St::NBD >= 3 ? 4 : 3;
