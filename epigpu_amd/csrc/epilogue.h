@@ -241,6 +241,28 @@ __device__ __forceinline__ bool screen_fast_maybe(const float n[9], const float 
     return dfok && (guard || f1 || f2);
 }
 
+// screen_fast_maybe without per-cell selects: nfac = sum min(n_c, 1), and
+// an empty cell's term is s_c^2 / max(n_c, 1), where s_c is zero but for FP32
+// rounding of the inclusion-exclusion; such a residue only raises SSB (and
+// lowers SSW), so the result is true wherever screen_fast_maybe's is.
+__device__ __forceinline__ bool screen_fast_lane(const float n[9], const float s[9], float N, float T2N, float sq,
+                                                 int test, int safe, float t1s, float t2s)
+{
+    float nfac = 0.0f, ssb = -T2N;
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+        nfac += fminf(n[c], 1.0f);
+        ssb = __builtin_fmaf(s[c] * s[c], __builtin_amdgcn_rcpf(fmaxf(n[c], 1.0f)), ssb);
+    }
+    const float d1 = nfac - 1.0f, d2 = N - nfac;
+    const bool dfok = (d1 >= 1.0f) & (d2 >= 1.0f) & ((test == 'f') | (safe != 0) | (nfac > 6.0f));
+    const float ssw = sq - T2N - ssb;
+    const bool guard = !(ssw > 1e-4f * (sq + T2N + ssb));
+    const bool f1 = ssb * d2 > t1s * d1 * ssw;
+    const bool f2 = (test != 'f') & (0.25f * ssb * d2 > t2s * ssw);
+    return dfok & (guard | f1 | f2);
+}
+
 // screen_fast_maybe for a pair whose nine cells are all occupied (nfac = 9,
 // d1 = 8, d2 = N - 9 >= 1, which the caller checks): with c1 = 8 t1s / d2 and
 // c2 = 4 t2s / d2 (c2 < 0 for test f, so that its test never holds), F > t1s

@@ -726,9 +726,13 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
         // pair does the wave take the per-pair path (full screen, FP64
         // decision, hit append) for it.
         const float Nf = (float)A.nV, T2N = qVf * qVf / Nf;
-        // Pairs with all nine cells occupied (nearly all of them) take
-        // screen_fast9, whose constants are the scan's; a wave with any other
-        // pair takes screen_fast_maybe.
+        // 64-row tiles: a wave whose pairs all have nine occupied cells takes
+        // screen_fast9, whose constants are the scan's; any other wave takes
+        // screen_fast_maybe. 32-row tiles (which the autotuner picks for
+        // batches of small scans, where empty cells are common) take
+        // screen_fast_lane for every pair. Each choice measured faster on its
+        // tiles (a scan of N = 8000, M = 50000; a batch of 64 scans with N
+        // from 300 to 6000).
         const float d2_9 = Nf - 9.0f;
         const bool all9_ok = d2_9 >= 1.0f;
         const float c1 = 8.0f * hs.t1s / d2_9, c2 = hs.test != 'f' ? 4.0f * hs.t2s / d2_9 : -1.0f;
@@ -751,14 +755,18 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
                     cells_from_x(xn, nf);
                     cells_from_x(xf, sf);
                     const bool valid = (a < a1) & (b < b1) & (a > b);
-                    float nmin = nf[0];
-#pragma unroll
-                    for (int c = 1; c < 9; c++) nmin = fminf(nmin, nf[c]);
                     bool keep;
-                    if (all9_ok && __ballot(!(nmin > 0.0f)) == 0ull)
-                        keep = valid & screen_fast9(nf, sf, T2N, SST, sqV, c1, c2);
-                    else
-                        keep = valid & screen_fast_maybe(nf, sf, Nf, T2N, sqV, hs.test, hs.safe, hs.t1s, hs.t2s);
+                    if constexpr (TA == 32) {
+                        keep = valid & screen_fast_lane(nf, sf, Nf, T2N, sqV, hs.test, hs.safe, hs.t1s, hs.t2s);
+                    } else {
+                        float nmin = nf[0];
+#pragma unroll
+                        for (int c = 1; c < 9; c++) nmin = fminf(nmin, nf[c]);
+                        if (all9_ok && __ballot(!(nmin > 0.0f)) == 0ull)
+                            keep = valid & screen_fast9(nf, sf, T2N, SST, sqV, c1, c2);
+                        else
+                            keep = valid & screen_fast_maybe(nf, sf, Nf, T2N, sqV, hs.test, hs.safe, hs.t1s, hs.t2s);
+                    }
                     if (__ballot(keep) != 0ull) finish(x, keep);
                     // keep the scheduler from interleaving the temporaries of several pairs
                     __builtin_amdgcn_sched_barrier(0);
