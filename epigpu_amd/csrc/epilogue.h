@@ -180,40 +180,6 @@ __device__ __forceinline__ bool screen32(const int n[9], const float s[9], float
     return 0.25f * ssi * (float)d2 > t2s * ssw;
 }
 
-// Branch-free first stage of screen32: false only if screen32 is false for
-// certain, i.e. the pair fails the degrees-of-freedom and nfac rules, or it
-// clears neither the SSW guard, nor the F threshold, nor the F2 bound (F2 <=
-// d1 F / 4, see screen32). Straight-line code, so that a lane's pairs can be
-// screened side by side; the few pairs it keeps go through screen32.
-__device__ __forceinline__ bool screen_maybe(const int n[9], const float s[9], float sq, int test, int safe,
-                                             float t1s, float t2s)
-{
-    int N = 0, nfac = 0;
-    float tot = 0.0f;
-#pragma unroll
-    for (int c = 0; c < 9; c++) {
-        N += n[c];
-        tot += s[c];
-        nfac += n[c] > 0;
-    }
-    const int d1 = nfac - 1, d2 = N - nfac;
-    const bool dfok = d1 >= 1 && d2 >= 1 && (test == 'f' || safe || nfac > 6);
-    const float rN = __builtin_amdgcn_rcpf((float)N);
-    const float mbar = tot * rN;
-    float ssb = 0.0f;
-#pragma unroll
-    for (int c = 0; c < 9; c++) {
-        const float m = n[c] > 0 ? s[c] * __builtin_amdgcn_rcpf((float)n[c]) : 0.0f;
-        const float e = m - mbar;
-        ssb += (float)n[c] * e * e;
-    }
-    const float ssw = sq - (float)N * mbar * mbar - ssb;
-    const bool guard = !(ssw > 1e-4f * (sq + (float)N * mbar * mbar + ssb));
-    const bool f1 = ssb * (float)d2 > t1s * (float)d1 * ssw;
-    const bool f2 = test != 'f' && 0.25f * ssb * (float)d2 > t2s * ssw;
-    return dfok && (guard || f1 || f2);
-}
-
 // First stage of the screen for FAST tiles, where every present individual is
 // a complete case, so that N, the phenotype total T and SST = sq - T^2 / N are
 // the same for every pair: SSB = sum_c s_c^2 / n_c - T^2 / N needs one
