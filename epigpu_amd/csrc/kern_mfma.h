@@ -658,6 +658,10 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
         x.bl = TA + 32 * bg + m;
         return x;
     };
+    // The outlier count, read once: the scan descriptor may sit in global
+    // memory (the batch kernel), where each slot's possible hit append would
+    // otherwise make the compiler reload it.
+    const int nout = __builtin_amdgcn_readfirstlane(A.nout);
     // FP32 sums of the pair's 3 x 3 table: the limbs, the outliers of the
     // cells {0,1} x {0,1} (all of them complete cases in a FAST tile) and the
     // marginals.
@@ -667,7 +671,7 @@ __device__ __forceinline__ void mfma_fast_tile(const MfmaScan &A, const HitSink 
 #pragma unroll
             for (int h = 0; h < 2; h++)
                 xf[g][h] = (float)x.l[g][h][0] + 256.0f * (float)x.l[g][h][1] + 65536.0f * (float)x.l[g][h][2];
-        if (A.nout) add_outlier_cells(A, a0 + x.al, b0 + x.bl - TA, xf);
+        if (nout) add_outlier_cells(A, a0 + x.al, b0 + x.bl - TA, xf);
 #pragma unroll
         for (int g = 0; g < 2; g++) {
             xf[g][2] = mg_f[2 * x.al + g];
