@@ -391,9 +391,12 @@ __device__ __forceinline__ void k_loop(St &sg, uint8_t *lds, int stage_bytes, in
     // tiles.) A scheduling fence closes each step, so that the compiler
     // cannot hoist the next step's ANDs (whose operands are still in flight
     // from LDS) into this step's MFMAs; within a step, each MFMA is followed
-    // by up to VS VALU instructions and one LDS read. VS follows the staging load:
+    // by up to VS VALU instructions and LDS reads. VS follows the staging load:
     // three for two dwords per thread and stage (64-row tiles), four for three
     // (32-row tiles), the fastest of 2 to 5 for each on the same two workloads.
+    // The next step's LDS reads: one after each MFMA for 64-row tiles, two
+    // after each of the first half of the MFMAs for 32-row tiles (likewise
+    // measured).
     // Masks: 0x008 MFMA, 0x002 VALU, 0x100 LDS read.
     constexpr int H = (St::NBD + 1) / 2;
     constexpr int VS = St::NBD >= 3 ? 4 : 3;
@@ -402,7 +405,10 @@ __device__ __forceinline__ void k_loop(St &sg, uint8_t *lds, int stage_bytes, in
         for (int u = 0; u < NMF; u++) {
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
             __builtin_amdgcn_sched_group_barrier(0x002, VS, 0);
-            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+            if (St::NBD < 3)
+                __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+            else if (u < NMF / 2)
+                __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
         }
         __builtin_amdgcn_sched_barrier(0);
     };
