@@ -261,8 +261,6 @@ struct Stager {
     static_assert(NWR % 4 == 0 && NWR * KC / 8 == 64 * (NWR / 4), "weight rows per wave");
     static constexpr bool PART = NB * NT > NDW;         // the last dword round leaves lanes idle
     static constexpr int NBD = NB;                      // dwords per thread per stage
-    static constexpr int NLD = NB + 1;                  // global loads per stage (dwords, weights)
-    static constexpr int NST = 2 * NB + 1;              // LDS writes per stage
     static_assert((NDW - (NB - 1) * NT) % 64 == 0, "idle lanes fill whole waves");
     bool pact;                                          // this wave has a dword in the last round
     const uint32_t *src[NB];
@@ -478,26 +476,39 @@ __device__ __forceinline__ void k_loop1(St &sg, uint8_t *lds, int stage_bytes, i
     sg.template store<0>(lds);
     sg.template load<0>(2 * KC < Kl ? 2 * KC : 0);
     __syncthreads();
+    // As in k_loop, a scheduling fence closes each step, and within a step each
+    // MFMA is followed by up to five VALU instructions and one LDS read; the
+    // next stage's expansion and LDS writes are split over steps 1 and 2, its
+    // global loads follow in step 2. Measured against one scheduling region
+    // per stage: GENERAL tiles of a scan of N = 4000, M = 20000 2.0 % faster,
+    // of a four-phenotype batch 1.0 % (three or four VALU slots: 2.8 / 2.3 %
+    // and 0.3 / 0.0 %).
+    auto step_end = [&]() {
+#pragma unroll
+        for (int u = 0; u < NMF; u++) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    };
     auto body = [&](auto tag, const uint8_t *cur, uint8_t *nxt, int kc) {
         constexpr int S = decltype(tag)::v;
         issue(o);  // step 3 of the previous stage (zero operands before the first)
         fetch(cur, 0, o);
+        step_end();
         issue(o);
         fetch(cur, 1, o);
+        sg.template store<S, 0, (St::NBD + 1) / 2, false>(nxt);
+        step_end();
         issue(o);
         fetch(cur, 2, o);
-        sg.template store<S>(nxt);
+        sg.template store<S, (St::NBD + 1) / 2, St::NBD, true>(nxt);
         sg.template load<S>(kc + 3 * KC < Kl ? kc + 3 * KC : Kl - KC);
+        step_end();
         issue(o);
         fetch(cur, 3, o);
-#pragma unroll
-        for (int u = 0; u < 4 * NMF; u++) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-            __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);
-            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-            if (u % 4 == 2 && u / 4 < St::NST) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
-            if (u >= 4 && u < 4 + St::NLD) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
-        }
+        step_end();
         // the next stage was written by every wave during this one
         __syncthreads();
     };
