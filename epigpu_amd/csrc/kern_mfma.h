@@ -476,19 +476,21 @@ __device__ __forceinline__ void k_loop1(St &sg, uint8_t *lds, int stage_bytes, i
     sg.template store<0>(lds);
     sg.template load<0>(2 * KC < Kl ? 2 * KC : 0);
     __syncthreads();
-    // As in k_loop, a scheduling fence closes each step, and within a step each
-    // MFMA is followed by up to five VALU instructions and one LDS read; the
-    // next stage's expansion and LDS writes are split over steps 1 and 2, its
+    // As in k_loop, a scheduling fence closes each step. Within a step each
+    // MFMA is followed by up to five VALU instructions, and the first half of
+    // the MFMAs by two LDS reads each (the next step's operands); the next
+    // stage's expansion and LDS writes are split over steps 1 and 2, its
     // global loads follow in step 2. Measured against one scheduling region
-    // per stage: GENERAL tiles of a scan of N = 4000, M = 20000 2.0 % faster,
-    // of a four-phenotype batch 1.0 % (three or four VALU slots: 2.8 / 2.3 %
-    // and 0.3 / 0.0 %).
+    // per stage: GENERAL tiles of a scan of N = 4000, M = 20000 2.0 % faster
+    // with one read after each MFMA, 4.5 % with the reads front-loaded; a
+    // four-phenotype batch 1.0 and 1.9 % (three or four VALU slots: 2.8 / 2.3 %
+    // and 0.3 / 0.0 %, one read per MFMA).
     auto step_end = [&]() {
 #pragma unroll
         for (int u = 0; u < NMF; u++) {
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
             __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);
-            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+            if (u < NMF / 2) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
         }
         __builtin_amdgcn_sched_barrier(0);
     };
